@@ -1,0 +1,145 @@
+"""Data sources behind esr.Tensor / esr.Selector / esr.Reducer.
+
+Follows the roles of easier/core/runtime/data_loader.py (DataLoaderBase
+:61-313, InMemoryTensorLoader :316-403, FulledTensorLoader :754-830,
+ArangeTensorLoader :833-931): a loader is a lightweight description
+(shape, dtype, device) until esr.compile() loads it. There is a single
+partition, so a loader only ever loads fully, onto the host, and the compiler
+permutes and uploads the result.
+"""
+import copy
+from typing import Optional, Tuple, Union
+
+import numpy as np
+import torch
+
+from .utils import EasierJitException
+
+ATTRIBUTE_PLACEHOLDER = "easier_placeholder"
+
+
+class DataLoaderBase:
+    shape: Tuple[int, ...]
+    dtype: torch.dtype
+    device: torch.device
+
+    def to(self, *, dtype: Optional[torch.dtype] = None,
+           device: Optional[Union[torch.device, str]] = None
+           ) -> 'DataLoaderBase':
+        clone = copy.copy(self)
+        if dtype is not None:
+            clone.dtype = dtype
+        if device is not None:
+            clone.device = torch.device(device)
+        return clone
+
+    def load_cpu(self) -> torch.Tensor:
+        """The full tensor on the host, in self.dtype."""
+        raise NotImplementedError
+
+    def minmax(self):
+        t = self.load_cpu()
+        if t.numel() == 0:
+            return 0, -1
+        return t.min().item(), t.max().item()
+
+    def get_placeholder(self, device=None) -> torch.Tensor:
+        """A zero-memory stand-in with the right shape/dtype
+        (data_loader.py:249-300): a size-1 buffer expanded to `shape`."""
+        if device is None:
+            device = self.device
+        if len(self.shape) == 0:
+            ph = torch.zeros((), dtype=self.dtype, device=device)
+        else:
+            ph = torch.zeros((1,), dtype=self.dtype,
+                             device=device).expand(self.shape)
+
+        def _to_forbidden(self, *args, **kwargs):
+            raise EasierJitException("Cannot call .to() on placeholders!")
+        ph.to = _to_forbidden.__get__(ph)
+        setattr(ph, ATTRIBUTE_PLACEHOLDER, True)
+        return ph
+
+
+class InMemoryTensorLoader(DataLoaderBase):
+    def __init__(self, tensor: torch.Tensor) -> None:
+        self.dtype = tensor.dtype
+        self.shape = tuple(tensor.shape)
+        self.device = tensor.device
+        self.tensor = tensor.detach().cpu()
+
+    def load_cpu(self):
+        return self.tensor.to(self.dtype)
+
+    def __repr__(self):
+        return f'{self.__class__.__name__}(tensor={self.tensor})'
+
+
+class FulledTensorLoader(DataLoaderBase):
+    def __init__(self, value, shape, dtype, device) -> None:
+        self.value = value
+        self.shape = tuple(shape)
+        self.dtype = dtype
+        self.device = torch.device(device)
+
+    def load_cpu(self):
+        return torch.full(self.shape, self.value, dtype=self.dtype)
+
+    def minmax(self):
+        return self.value, self.value
+
+    def __repr__(self):
+        return (f'{self.__class__.__name__}(value={self.value}, '
+                f'shape={self.shape}, dtype={self.dtype})')
+
+
+class ArangeTensorLoader(DataLoaderBase):
+    def __init__(self, start, end, step, dtype, device) -> None:
+        if step == 0:
+            raise ValueError("step must not be 0")
+        self.start, self.end, self.step = start, end, step
+        self.dtype = dtype
+        self.device = torch.device(device)
+        length = len(range(0, int(np.ceil((end - start) / step))))
+        self.shape = (max(0, length),)
+
+    def load_cpu(self):
+        return torch.arange(self.start, self.end, self.step, dtype=self.dtype)
+
+    def __repr__(self):
+        return (f'{self.__class__.__name__}(start={self.start}, '
+                f'end={self.end}, step={self.step}, dtype={self.dtype})')
+
+
+class H5DataLoader(DataLoaderBase):
+    """A dataset in an HDF5 file (the mesh schema of
+    tutorial/create_triangular_mesh.py:29-35). Reading needs h5py; the shape
+    and dtype are read when the loader is created."""
+
+    def __init__(self, file: str, dataset: str, dtype=None, device='cpu',
+                 **h5_file_kwargs) -> None:
+        try:
+            import h5py  # noqa: F401
+        except ImportError as e:
+            raise ImportError(
+                'esr.hdf5 needs h5py, which is not installed; load the data '
+                'with numpy/torch and pass the tensor instead') from e
+        import h5py
+        self.file, self.dataset = file, dataset
+        self.h5_file_kwargs = h5_file_kwargs
+        with h5py.File(file, 'r', **h5_file_kwargs) as f:
+            d = f[dataset]
+            self.shape = tuple(d.shape)
+            raw = torch.from_numpy(np.empty((0,), dtype=d.dtype)).dtype
+        self.dtype = dtype if dtype is not None else raw
+        self.device = torch.device(device)
+
+    def load_cpu(self):
+        import h5py
+        with h5py.File(self.file, 'r', **self.h5_file_kwargs) as f:
+            arr = f[self.dataset][...]
+        return torch.from_numpy(np.ascontiguousarray(arr)).to(self.dtype)
+
+    def __repr__(self):
+        return (f'{self.__class__.__name__}(file={self.file!r}, '
+                f'dataset={self.dataset!r}, dtype={self.dtype})')

@@ -1,0 +1,222 @@
+/*
+ * esr_hip.h -- C ABI of libesrhip.so, the MI355X (gfx950) backend of the
+ * EASIER programming model (esr.Module / esr.Selector / esr.Reducer /
+ * esr.compile, easier.numeric CG/GMRES).
+ *
+ * Contract for every entry point:
+ *   - returns 0 on success, a positive ESR_ERR_* code otherwise;
+ *     esr_last_error() then returns a thread-local message;
+ *   - no C++ exception crosses this boundary;
+ *   - all device buffers are owned by the caller (plain device pointers and
+ *     sizes, no framework types); the library allocates device memory only
+ *     inside objects it hands back (compiled modules, graph executables), each
+ *     with an explicit destroy call;
+ *   - every device launch goes on the caller's stream (`stream` is a
+ *     hipStream_t passed as void*; NULL means the null stream);
+ *   - index arrays are int32 on the device; their ranges are validated by the
+ *     host before any launch (kernels do not bounds-check indices).
+ *
+ * Each declaration cites the reference interface (file:line in
+ * jack-pan-ai/EASIER) whose behaviour it replaces.
+ */
+#ifndef ESR_HIP_H
+#define ESR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define ESR_OK 0
+#define ESR_ERR_INVALID 1     /* bad argument (shape, dtype, size)          */
+#define ESR_ERR_HIP 2         /* HIP runtime error                          */
+#define ESR_ERR_UNSUPPORTED 3 /* valid request this build does not handle   */
+#define ESR_ERR_RTC 4         /* runtime compilation failed (log in error)  */
+#define ESR_ERR_INTERNAL 5    /* unexpected internal failure                */
+
+/* ---- dtype codes --------------------------------------------------------- */
+#define ESR_F32 0
+#define ESR_F64 1
+#define ESR_I32 2
+#define ESR_I64 3
+#define ESR_U8 4 /* torch.bool / uint8 */
+#define ESR_BF16 5
+
+/* ---- Reducer ops: easier/core/module.py:386 (ReduceOp) ------------------- */
+#define ESR_RED_SUM 0
+#define ESR_RED_PROD 1
+#define ESR_RED_MEAN 2
+#define ESR_RED_AMAX 3
+#define ESR_RED_AMIN 4
+
+/* ---- aggregators: easier/core/module.py:825-884 -------------------------- */
+#define ESR_AGG_SUM 0
+#define ESR_AGG_PROD 1
+#define ESR_AGG_NORM 2 /* p-norm, p passed separately (inf allowed)        */
+#define ESR_AGG_MAX 3
+#define ESR_AGG_MIN 4
+
+/* ---- library ------------------------------------------------------------- */
+int esr_abi_version(void);
+const char *esr_last_error(void);
+/* gcnArchName of the current device, e.g. "gfx950:sramecc+:xnack-". */
+int esr_device_arch(char *buf, int buflen);
+/* Blocks until all work on `stream` is done. */
+int esr_stream_sync(void *stream);
+
+/* ---- a1: Selector.forward  (easier/core/module.py:364-371) ---------------
+ * dst[e, :] = src[idx[e], :] for e < n_out; a row is `width` elements of
+ * `dtype`. Pure data movement: bit-exact.                                    */
+int esr_gather(int dtype, const void *src, int64_t n_src, int64_t width,
+               const int32_t *idx, int64_t n_out, void *dst, void *stream);
+
+/* ---- a2: Reducer.forward  (easier/core/module.py:441-461) -----------------
+ * out[r, :] = reduce_{j in [rowptr[r], rowptr[r+1])} vals[perm ? perm[j] : j, :]
+ * i.e. torch scatter_reduce_(0, idx, vals, op, include_self=False) on a CSR
+ * view of idx (rows = distinct targets, entries in original edge order).
+ * Rows of at most ESR_SEQ_ROW_MAX entries are reduced sequentially in entry
+ * order from the op's identity (bit-identical to torch CPU); longer rows, listed
+ * in long_rows[0..n_long), use a fixed-shape tree (deterministic, not
+ * bit-identical). keep_empty != 0: rows with no entries keep `out` (the
+ * `out=` argument); keep_empty == 0: they are set to 0.                      */
+#define ESR_SEQ_ROW_MAX 256
+int esr_csr_reduce(int op, int dtype, const void *vals, int64_t width,
+                   const int32_t *rowptr, const int32_t *perm, int64_t n_rows,
+                   const int32_t *long_rows, int64_t n_long, int keep_empty,
+                   void *out, void *stream);
+
+/* ---- a4: aggregators esr.sum/prod/norm/max/min (module.py:825-884, with the
+ * single-partition rewrite of passes/dataflow_distribution.py:197-289) ------
+ * out[0, c] = agg_{i<n} x[i, c] for c < width (x row-major (n, width)).
+ * Two fixed-shape stages through `ws` (esr_aggregate_ws_bytes bytes):
+ * deterministic run to run.                                                  */
+int esr_aggregate_ws_bytes(int dtype, int64_t n, int64_t width, int64_t *bytes);
+int esr_aggregate(int op, int dtype, const void *x, int64_t n, int64_t width,
+                  double p, void *out, void *ws, int64_t ws_bytes,
+                  void *stream);
+
+/* ---- elementwise operators of a traced esr.Module forward ----------------
+ * Executes one torch elementwise operator (the call_function/call_method
+ * nodes evaluated by runtime/jit_engine/values.py:330-383 in the reference)
+ * over an up-to-6-d strided iteration space.                                 */
+#define ESR_EW_MAXDIM 6
+/* unary */
+#define ESR_EW_COPY 0 /* cast/copy                                          */
+#define ESR_EW_NEG 1
+#define ESR_EW_ABS 2
+#define ESR_EW_EXP 3
+#define ESR_EW_LOG 4
+#define ESR_EW_SQRT 5
+#define ESR_EW_RSQRT 6
+#define ESR_EW_SIN 7
+#define ESR_EW_COS 8
+#define ESR_EW_TANH 9
+#define ESR_EW_SIGN 10
+#define ESR_EW_RECIP 11
+#define ESR_EW_FLOOR 12
+#define ESR_EW_CEIL 13
+#define ESR_EW_NOT 14
+#define ESR_EW_SIGMOID 15
+#define ESR_EW_RELU 16
+#define ESR_EW_ISNAN 17
+/* binary */
+#define ESR_EW_ADD 32
+#define ESR_EW_SUB 33
+#define ESR_EW_MUL 34
+#define ESR_EW_DIV 35
+#define ESR_EW_POW 36
+#define ESR_EW_MAXIMUM 37
+#define ESR_EW_MINIMUM 38
+#define ESR_EW_LT 39
+#define ESR_EW_LE 40
+#define ESR_EW_GT 41
+#define ESR_EW_GE 42
+#define ESR_EW_EQ 43
+#define ESR_EW_NE 44
+#define ESR_EW_AND 45
+#define ESR_EW_OR 46
+#define ESR_EW_ATAN2 47
+#define ESR_EW_FMOD 48
+#define ESR_EW_FLOORDIV 49
+/* ternary */
+#define ESR_EW_WHERE 64 /* in0 (u8) ? in1 : in2                            */
+
+/* out and every tensor input share `shape` (ndim <= 6) through their own
+ * element strides (stride 0 = broadcast). Tensor inputs all have dtype
+ * `in_dtype`; an input with a NULL pointer is the host scalar scalars[k]
+ * (cast to in_dtype). For WHERE, input 0 is u8 and inputs 1, 2 are in_dtype.
+ * out_dtype is in_dtype, or ESR_U8 for comparisons/ISNAN/NOT/AND/OR, or any
+ * dtype for COPY (a cast).                                                   */
+int esr_ewise(int op, int ndim, const int64_t *shape, int out_dtype, void *out,
+              const int64_t *out_strides, int in_dtype, int n_in,
+              void *const *in_ptrs, const int64_t *in_strides,
+              const double *scalars, void *stream);
+int esr_fill(int dtype, void *dst, int64_t n, double value, void *stream);
+
+/* y[r] = sum_{k<kdim} A[r*lda + k*ldk] * x[k*incx] summed in k order, for
+ * r < n: the column-block product torch.matmul(V[:, :i], y[:i]) of
+ * easier/numeric/solver/gmres.py:133-136 (UpdateX).                          */
+int esr_gemv_rows(int dtype, const void *A, int64_t n, int64_t kdim,
+                  int64_t lda, int64_t ldk, const void *x, int64_t incx,
+                  void *y, void *stream);
+
+/* ---- runtime compilation and launch of fused kernels ---------------------
+ * Replaces the reference's per-node interpreter (jit_engine.py:1114-1172) by
+ * fused kernels compiled once per esr.compile() (hipRTC, gfx950).            */
+int esr_rtc_compile(const char *source, const char *name, const char *options,
+                    void **module_out);
+int esr_rtc_function(void *module, const char *kernel_name, void **fn_out);
+int esr_rtc_unload(void *module);
+int esr_launch(void *fn, uint32_t gx, uint32_t gy, uint32_t gz, uint32_t bx,
+               uint32_t by, uint32_t bz, uint32_t shmem, void *stream,
+               void *argbuf, size_t argbytes);
+
+/* ---- HIP graphs (capture of a fixed launch sequence) --------------------- */
+int esr_graph_begin(void *stream);
+int esr_graph_end(void *stream, void **exec_out);
+int esr_graph_launch(void *exec, void *stream);
+int esr_graph_destroy(void *exec);
+
+/* ---- a9: mesh generators (host) -------------------------------------------
+ * esr_mesh_tri4: csrc/triangular_mesh.cpp:10-142 (4 triangles per square
+ * around its centre point; same arrays, same numbering, same edge order).
+ * esr_mesh_tri2: the same six arrays for 2 triangles per square (diagonal
+ * from (i,j) to (i+1,j+1)), optionally with interior points jittered.
+ * Sizes: esr_mesh_sizes(kind, n, &np, &nc, &ne, &nbc), kind 4 or 2.
+ * Arrays (caller-allocated, host memory): src/dst (ne) int64, cells (nc*3)
+ * int64, bcells (nbc) int64, bpoints (nbc*2) int64, points (np*2) double.   */
+int esr_mesh_sizes(int kind, int64_t n, int64_t *np_, int64_t *nc,
+                   int64_t *ne, int64_t *nbc);
+int esr_mesh_tri4(int64_t n, int64_t *src, int64_t *dst, int64_t *cells,
+                  int64_t *bcells, int64_t *bpoints, double *points);
+int esr_mesh_tri2(int64_t n, double jitter, uint64_t seed, int64_t *src,
+                  int64_t *dst, int64_t *cells, int64_t *bcells,
+                  int64_t *bpoints, double *points);
+
+/* ---- a10: index renumbering (host) -----------------------------------------
+ * Replaces the world-size-1 outcome of passes/sparse_encoding/
+ * sparse_encoding.py:482-595 (edges stable-sorted by the bound Reducer's
+ * idx) and adds a locality renumbering of the vertex/cell space.
+ *
+ * esr_csr_from_idx: rowptr[n+1] and perm[m] such that perm lists the entries
+ * of idx grouped by row, rows ascending, entries of one row in input order
+ * (a stable counting sort). Fails if any idx is outside [0, n).               */
+int esr_csr_from_idx(const int64_t *idx, int64_t m, int64_t n, int64_t *rowptr,
+                     int64_t *perm);
+/* Reverse Cuthill-McKee order of the symmetric graph given by the directed
+ * pair list (a[k], b[k]), k < m, on n vertices (self loops ignored; each
+ * connected component started from a pseudo-peripheral vertex).
+ * order[new] = old.                                                           */
+int esr_rcm(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
+            int64_t *order);
+/* Largest |row(e) - col(e)| over the pairs: the bandwidth RCM minimises.     */
+int esr_bandwidth(const int64_t *a, const int64_t *b, int64_t m,
+                  int64_t *bw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ESR_HIP_H */

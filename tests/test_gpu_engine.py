@@ -1,0 +1,152 @@
+"""End-to-end parity of compiled programs (esr.compile(..., 'hip')) against
+the oracle: Linsys/CG/GMRES, the shallow-water assembly programs and RK4
+steps, Reducer `out=` semantics and Tensor.collect() ordering."""
+import numpy as np
+import pytest
+import torch
+
+import easier_amd as esr
+from easier_amd.numeric import Linsys
+from easier_amd.numeric.preconditioner import Jacobi
+from easier_amd.numeric.solver import CG, GMRES
+from easier_amd.workloads import mesh as wmesh
+from easier_amd.workloads import swe as wswe
+from oracle import ops as oops
+from oracle import solvers as osol
+from oracle import swe as oswe
+
+pytestmark = pytest.mark.gpu
+
+
+def _laplacian_edges(m):
+    rows, cols = [], []
+    for j in range(m):
+        for i in range(m):
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                a, b = i + di, j + dj
+                if 0 <= a < m and 0 <= b < m:
+                    rows.append(j * m + i)
+                    cols.append(b * m + a)
+    return torch.tensor(rows), torch.tensor(cols)
+
+
+class Poisson5(esr.Module):
+    """5-point Laplacian + diagonal shift as a Linsys (tutorial/poisson
+    layout: Reducer(rows), Selector(cols), Af per edge, Ac per vertex)."""
+
+    def __init__(self, m, seed=0):
+        super().__init__()
+        rows, cols = _laplacian_edges(m)
+        n = m * m
+        g = torch.Generator().manual_seed(seed)
+        self.rows, self.cols = rows, cols
+        self.reducer = esr.Reducer(rows, n)
+        self.selector = esr.Selector(cols)
+        self.x = esr.Tensor(esr.zeros((n,), dtype=torch.double), mode='partition')
+        self.b = esr.Tensor(torch.rand(n, generator=g, dtype=torch.double) * 2 - 1,
+                            mode='partition')
+        self.Ac = esr.Tensor(4.0 + torch.rand(n, generator=g, dtype=torch.double),
+                             mode='partition')
+        self.Af = esr.Tensor(-torch.ones(rows.shape[0], dtype=torch.double),
+                             mode='partition')
+        self.A = Linsys(self.Ac, self.Af, self.selector, self.reducer)
+
+    def oracle(self):
+        n = self.Ac.easier_data_loader.shape[0]
+        return osol.Linsys(self.Ac.easier_data_loader.load_cpu(),
+                           self.Af.easier_data_loader.load_cpu(),
+                           self.cols, self.rows, n), \
+            self.b.easier_data_loader.load_cpu()
+
+
+@pytest.mark.parametrize('precond', [False, True])
+@pytest.mark.parametrize('reorder', ['rcm', 'none'])
+def test_cg_matches_oracle(gpu, precond, reorder):
+    print(f'cg precond={precond} reorder={reorder}', flush=True)
+    eqn = Poisson5(24)
+    A_o, b_o = eqn.oracle()
+    M = Jacobi(eqn.Ac) if precond else None
+    sol = CG(eqn.A, eqn.b, eqn.x, M)
+    esr.compile([eqn, sol], 'hip', reorder=reorder)
+    info = sol.solve(rtol=1e-10, maxiter=2000)
+    x_o = torch.zeros_like(b_o)
+    info_o, x_o = osol.cg(A_o, b_o, x_o, M=(lambda r: r / A_o.Av) if precond else None,
+                          rtol=1e-10, maxiter=2000)
+    assert abs(info['iters'] - info_o['iters']) <= 1
+    assert info['residual'] <= 1e-10 * float(torch.norm(b_o)) * 1.0000001
+    torch.testing.assert_close(eqn.x.collect(), x_o, rtol=1e-8, atol=1e-10)
+
+
+def test_gmres_matches_oracle(gpu):
+    eqn = Poisson5(12, seed=1)
+    A_o, b_o = eqn.oracle()
+    sol = GMRES(eqn.A, eqn.b, eqn.x, restart=10)
+    esr.compile([eqn, sol], 'hip')
+    info = sol.solve(rtol=1e-9, maxiter=200)
+    x_o = torch.zeros_like(b_o)
+    info_o, x_o = osol.gmres(A_o, b_o, x_o, restart=10, rtol=1e-9, maxiter=200)
+    assert abs(info['iters'] - info_o['iters']) <= 1
+    torch.testing.assert_close(eqn.x.collect(), x_o, rtol=1e-7, atol=1e-9)
+    assert info['residual'] < 1e-9 * float(torch.norm(b_o)) * 1.0001
+
+
+def test_reducer_out_and_collect_in_program(gpu):
+    class M(esr.Module):
+        def __init__(self):
+            super().__init__()
+            idx = torch.tensor([1, 1, 2, 2, 2])
+            self.red = esr.Reducer(idx, 3)
+            self.sel = esr.Selector(torch.tensor([2, 0, 1, 1, 0]))
+            g = torch.Generator().manual_seed(0)
+            self.v = esr.Tensor(torch.randn((3, 2), generator=g, dtype=torch.double),
+                                mode='partition')
+            self.e = esr.Tensor(torch.randn((5, 2), generator=g, dtype=torch.double),
+                                mode='partition')
+            self.o = esr.Tensor(torch.full((3, 2), 9.0, dtype=torch.double),
+                                mode='partition')
+
+        def forward(self):
+            self.red(self.sel(self.v) * self.e, out=self.o)
+
+    m = M()
+    v0, e0 = m.v.easier_data_loader.load_cpu(), m.e.easier_data_loader.load_cpu()
+    with pytest.raises(RuntimeError):
+        m.sel(v0)
+    esr.compile([m], 'hip')
+    m()
+    ref = oops.reducer(oops.selector(v0, torch.tensor([2, 0, 1, 1, 0])) * e0,
+                       torch.tensor([1, 1, 2, 2, 2]), 3,
+                       out=torch.full((3, 2), 9.0, dtype=torch.double))
+    assert torch.equal(m.o.collect(), ref)
+    with pytest.raises(esr.EasierJitException):
+        esr.compile([m], 'hip')
+
+
+@pytest.mark.parametrize('gen', ['tri4', 'tri2'])
+def test_swe_geometry_programs(gpu, gen):
+    mesh = getattr(wmesh, gen)(6)
+    geo = wswe.assemble_geometry(mesh, centre=(0.4, 0.6), width=30.0, amp=0.2)
+    ref = oswe.geometry(mesh, centre=(0.4, 0.6), width=30.0, amp=0.2)
+    for k in ('x', 'y', 'area', 'sx', 'sy', 'bsx', 'bsy', 'alpha'):
+        assert torch.equal(geo[k].cpu(), ref[k]), k
+    torch.testing.assert_close(geo['h'].cpu(), ref['h'], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('gen,g', [('tri4', 1.0), ('tri2', 9.81)])
+def test_swe_steps_match_oracle(gpu, dtype, gen, g):
+    print(f'swe {gen} g={g} {dtype}', flush=True)
+    mesh = getattr(wmesh, gen)(10)
+    geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
+    dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], g)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=g, dtype=dtype)
+    esr.compile([eqn], 'hip')
+    ref = oswe.SWE(mesh, geo, dt=dt, g=g, dtype=dtype)
+    for _ in range(20):
+        eqn()
+        ref()
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for k in ('h', 'uh', 'vh'):
+        got, want = getattr(eqn, k).collect(), getattr(ref, k)
+        scale = float(want.abs().max())
+        assert float((got - want).abs().max()) <= tol * scale, k
