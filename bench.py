@@ -1,0 +1,245 @@
+"""Benchmark: shallow-water RK4 steps on one MI355X (the hot path of
+esr.Module forwards: cell->edge Selector, flux, edge->cell Reducer, update).
+
+Workload (default): the 2-triangle mesh of a 2048 x 2048 cell grid
+(8,388,608 triangles, 25,157,632 directed cell-to-cell edges, 8,192 wall
+faces), state (h, hu, hv) in float64, h = 1 + 0.1 exp(-|x - x0|^2 / 0.005)
+with x0 ~ U[0.3, 0.7]^2 (seed 0), hu = hv = 0, g = 9.81, reflective walls,
+dt from CFL 0.4. The program is the tutorial's ShallowWaterEquation module
+compiled with esr.compile(..., 'hip'); a step is one forward() (one RK4 step).
+
+Prints one JSON line (rank 0, N = 1). Side figures: the P1 Poisson
+Jacobi-PCG solve of the 100 x 100 mesh (10,201 vertices, tol 1e-8).
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("Poisson solve time to 1e-8 rel. residual (learned vs Jacobi PCG); "
+          "SWE cells/s")
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(f'[bench {time.strftime("%H:%M:%S")}] {msg}', file=sys.stderr, flush=True)
+
+
+def swe_bytes(nc, ne, nbc, T, I=4):
+    """Algorithmic (compulsory) HBM bytes of one RK4 step of the fused
+    schedule: each of the 4 stage kernels reads per directed edge the source
+    index and (sx, sy, alpha), per wall face (bsx, bsy), two row pointers per
+    cell; the 55 cell-field reads/writes of the 4 stages (10 + 13 + 13 + 19
+    per cell: state, area, stage inputs, stage deltas, update)."""
+    per_stage_edges = ne * (I + 3 * T) + nbc * 2 * T + 2 * (nc + 1) * I
+    cells = {0: 10, 1: 13, 2: 13, 3: 19}
+    stages = [per_stage_edges + cells[k] * nc * T for k in range(4)]
+    return stages, sum(stages)
+
+
+def build_swe(n, dtype, seed, fuse=True):
+    import numpy as np
+    import torch
+    import easier_amd as esr
+    from easier_amd.workloads import mesh as wmesh
+    from easier_amd.workloads import swe as wswe
+    t0 = time.perf_counter()
+    mesh = wmesh.tri2(n)
+    log(f'mesh {n}x{n}: {mesh["cells"].shape[0]} cells, {mesh["src"].shape[0]} '
+        f'edges ({time.perf_counter() - t0:.1f} s)')
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x0 = tuple(float(v) for v in rng.uniform(0.3, 0.7, 2))
+    t0 = time.perf_counter()
+    geo = wswe.assemble_geometry(mesh, centre=x0, width=1.0 / 0.005, amp=0.1)
+    log(f'geometry programs on the GPU ({time.perf_counter() - t0:.1f} s)')
+    g = 9.81
+    dt = wswe.cfl_dt(mesh, geo['h'], g, cfl=0.4)
+    t0 = time.perf_counter()
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=g, dtype=dtype)
+    esr.compile([eqn], 'hip', fuse=fuse)
+    torch.cuda.synchronize()
+    log(f'compiled ({type(eqn.easier_engine).__name__}, '
+        f'{time.perf_counter() - t0:.1f} s), dt = {dt:.3e}')
+    return mesh, geo, eqn, dict(x0=x0, g=g, dt=dt)
+
+
+def time_kernels(eqn, reps):
+    """Average duration of each fused kernel, from HIP events on the stream
+    the kernels are launched on (the current stream)."""
+    import ctypes
+    import torch
+    from easier_amd import _native as nat
+    eng = eqn.easier_engine
+    if not hasattr(eng, 'launches'):
+        return None
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    totals = [0.0] * len(eng.launches)
+    for _ in range(reps):
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(eng.launches) + 1)]
+        evs[0].record(stream)
+        for k, (fn, grid, buf) in enumerate(eng.launches):
+            nat.call('esr_launch', fn, grid, 1, 1, 256, 1, 1, 0, sp,
+                     ctypes.addressof(buf), ctypes.sizeof(buf))
+            evs[k + 1].record(stream)
+        torch.cuda.synchronize()
+        for k in range(len(eng.launches)):
+            totals[k] += evs[k].elapsed_time(evs[k + 1]) / 1e3
+    return [t / reps for t in totals]
+
+
+def cpu_baseline_swe(mesh, geo, meta, dtype, steps):
+    """The oracle (torch CPU operators of the reference's CPU backend) on the
+    same mesh and state for a bounded number of RK4 steps."""
+    import torch
+    from oracle import swe as oswe
+    ref = oswe.SWE(mesh, geo, dt=meta['dt'], g=meta['g'], dtype=dtype)
+    ref()                                   # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ref()
+    dt = time.perf_counter() - t0
+    nc = mesh['cells'].shape[0]
+    return dict(value=nc * steps / dt, unit='cells/s', cores=torch.get_num_threads(),
+                kind='port', sample=f'{steps} RK4 steps of the same {nc}-cell '
+                f'mesh after 1 warm-up step, torch CPU operators',
+                seconds=dt)
+
+
+def poisson_config1(skip_cpu):
+    import torch
+    import easier_amd as esr
+    from easier_amd.numeric.solver import CG
+    from easier_amd.workloads import poisson as wp
+    mesh, sys_ = wp.config1_system(100, seed=0)
+    eqn = wp.PoissonP1(sys_)
+    sol = CG(eqn.A, eqn.b, eqn.x, eqn.M)
+    esr.compile([eqn, sol], 'hip')
+    res = None
+    times = []
+    for rep in range(4):
+        torch.cuda.synchronize()
+        eqn.x.data.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sol.solve(rtol=1e-8, maxiter=5000)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    out = dict(workload='P1 Poisson 100x100 (10,201 vertices), Jacobi-PCG, FP64, '
+               'rtol 1e-8', solve_ms=1e3 * sorted(times[1:])[1], iters=res['iters'],
+               residual=res['residual'])
+    if not skip_cpu:
+        from oracle import solvers as osol
+        A = osol.Linsys(torch.from_numpy(sys_['Av']), torch.from_numpy(sys_['Ae']),
+                        torch.from_numpy(sys_['col']), torch.from_numpy(sys_['row']),
+                        sys_['Av'].shape[0])
+        b = torch.from_numpy(sys_['b'])
+        ct = []
+        for rep in range(3):
+            x = torch.zeros_like(b)
+            t0 = time.perf_counter()
+            info, _ = osol.cg(A, b, x, M=lambda r: r / A.Av, rtol=1e-8, maxiter=5000)
+            ct.append(time.perf_counter() - t0)
+        out['cpu_solve_ms'] = 1e3 * sorted(ct)[1]
+        out['cpu_iters'] = info['iters']
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--n', type=int, default=2048)
+    ap.add_argument('--dtype', choices=['f64', 'f32'], default='f64')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--cpu-steps', type=int, default=2)
+    ap.add_argument('--skip-cpu', action='store_true')
+    ap.add_argument('--skip-poisson', action='store_true')
+    ap.add_argument('--no-fuse', action='store_true')
+    ap.add_argument('--profile', action='store_true',
+                    help='only warm-up + timed steps (for rocprofv3 runs)')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py: this build runs on one GPU; --gpus must be 1 '
+              '(no multi-GPU path is built)', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py: no GPU visible', file=sys.stderr)
+        sys.exit(2)
+    dtype = torch.float64 if args.dtype == 'f64' else torch.float32
+    T = 8 if dtype == torch.float64 else 4
+    mesh, geo, eqn, meta = build_swe(args.n, dtype, args.seed, fuse=not args.no_fuse)
+    nc, ne, nbc = mesh['cells'].shape[0], mesh['src'].shape[0], mesh['bcells'].shape[0]
+    for _ in range(args.warmup):
+        eqn()
+    torch.cuda.synchronize()
+    log(f'warm-up {args.warmup} steps done; timing {args.steps} steps')
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        eqn()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_s = start.elapsed_time(stop) / 1e3
+    ms_per_step = 1e3 * wall / args.steps
+    value = nc * args.steps / wall
+    log(f'{ms_per_step:.4f} ms/step (events {1e3 * gpu_s / args.steps:.4f}), '
+        f'{value:.3e} cells/s')
+    h = eqn.h.collect()
+    finite = bool(torch.isfinite(h).all())
+    stages, step_bytes = swe_bytes(nc, ne, nbc, T)
+    result = {
+        'metric': METRIC, 'value': value, 'unit': 'cells/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': args.dtype, 'data': 'synthetic',
+        'config': {
+            'workload': f'shallow-water RK4, 2-triangle mesh {args.n}x{args.n} '
+                        f'({nc} cells, {ne} directed edges, {nbc} wall faces), '
+                        f'{args.dtype}, g=9.81, CFL 0.4 (dt={meta["dt"]:.4e}), '
+                        'reflective walls',
+            'cells': nc, 'edges': ne, 'engine': type(eqn.easier_engine).__name__,
+            'step_bytes_algorithmic': step_bytes,
+            'step_hbm_fraction': step_bytes / (wall / args.steps) / HBM_PEAK,
+            'state_finite': finite,
+        },
+    }
+    if args.profile:
+        print(json.dumps(result), flush=True)
+        return
+    ktimes = time_kernels(eqn, reps=20)
+    if ktimes:
+        k = max(range(len(ktimes)), key=lambda i: ktimes[i])
+        achieved = stages[k] / ktimes[k]
+        result['roofline'] = {
+            'bound': 'hbm', 'kernel': f'esr_fused_{k} (RK4 stage {k + 1} of 4)',
+            'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+            'frac': achieved / HBM_PEAK, 'traffic': None,
+            'bytes_per_launch': stages[k], 'avg_launch_s': ktimes[k],
+            'all_kernels_s': ktimes,
+        }
+        log(f'dominant kernel esr_fused_{k}: {ktimes[k] * 1e6:.1f} us, '
+            f'{achieved / 1e9:.0f} GB/s ({achieved / HBM_PEAK:.1%} of 8 TB/s)')
+    if not args.skip_cpu:
+        log(f'CPU baseline: {args.cpu_steps} oracle steps on the host')
+        result['cpu_baseline'] = cpu_baseline_swe(mesh, geo, meta, dtype, args.cpu_steps)
+        result['cpu_baseline']['cpu_model'] = platform.processor() or platform.machine()
+    if not args.skip_poisson:
+        log('Poisson config 1')
+        result['poisson_config1'] = poisson_config1(args.skip_cpu)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -66,6 +66,7 @@ _SIGNATURES = {
     'esr_gemv_rows': (_c_i32, [_c_i32, _c_vp, _c_i64, _c_i64, _c_i64, _c_i64,
                                _c_vp, _c_i64, _c_vp, _c_vp]),
     'esr_rtc_compile': (_c_i32, [_c_cp, _c_cp, _c_cp, ctypes.POINTER(_c_vp)]),
+    'esr_rtc_check': (_c_i32, [_c_cp, _c_cp, _P_i64]),
     'esr_rtc_function': (_c_i32, [_c_vp, _c_cp, ctypes.POINTER(_c_vp)]),
     'esr_rtc_unload': (_c_i32, [_c_vp]),
     'esr_launch': (_c_i32, [_c_vp, _c_u32, _c_u32, _c_u32, _c_u32, _c_u32,
@@ -81,6 +82,9 @@ _SIGNATURES = {
     'esr_csr_from_idx': (_c_i32, [_c_vp, _c_i64, _c_i64, _c_vp, _c_vp]),
     'esr_rcm': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_i64, _c_vp]),
     'esr_bandwidth': (_c_i32, [_c_vp, _c_vp, _c_i64, _P_i64]),
+    'esr_p1_nnz': (_c_i32, [_c_i64, _c_i64, _c_vp, _P_i64]),
+    'esr_p1_assemble': (_c_i32, [_c_i64, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp,
+                                 _c_i64] + [_c_vp] * 6),
 }
 
 _LIB = None

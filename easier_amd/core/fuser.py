@@ -1,5 +1,677 @@
-"""Fused lowering of esr.Module forwards (filled in below)."""
+"""Fused lowering of esr.Module forwards into row kernels.
+
+The reference evaluates a forward node by node (runtime/jit_engine/
+jit_engine.py:1114-1172): every Selector materialises an edge-sized tensor,
+every elementwise op another, every Reducer a scatter. Here a forward is
+symbolically executed once into a value graph and scheduled into *row
+kernels*: a kernel iterates the rows of one index space (cells, vertices) and
+
+  - evaluates every Reducer writing that space as a loop over the row's CSR
+    range (edges sorted by row, so the loop is the reference's sequential
+    per-row sum in edge order), with the edge expression inlined: edge
+    tensors are read at the edge position, Selector outputs are read through
+    the Selector's index (or at the row itself when the Selector's index is
+    the Reducer's row index),
+  - evaluates the elementwise epilogue of the row,
+  - writes only what a later kernel, a Selector, or an assignment to an
+    esr.Tensor needs.
+
+A value gathered through a Selector must be complete before the gathering
+kernel starts, so gathers bump the consumer to the next stage; everything
+else fuses. An assignment to a tensor is placed after every kernel that reads
+the old contents at other rows. The kernels of one forward are generated as
+C++ from the template below, compiled once with hipRTC for the GPU, launched
+in stage order and captured as one HIP graph.
+
+Forwards using anything outside this subset (aggregators, views, replicated
+tensor updates, nested modules, multi-column tensors, ...) keep the
+node-by-node HipEngine.
+"""
+import ctypes
+import hashlib
+import operator
+import struct
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+from torch import fx
+
+from .. import _native as nat
+from . import module as _M
+from .compiler import CompiledProgram, resolve_attr
+from .utils import logger
 
 
-def try_fuse_program(prog):
-    return {}
+class Unsupported(Exception):
+    pass
+
+
+_EW_BIN = {
+    operator.add: 'add', operator.sub: 'sub', operator.mul: 'mul',
+    operator.truediv: 'div', operator.pow: 'pow',
+    operator.lt: 'lt', operator.le: 'le', operator.gt: 'gt', operator.ge: 'ge',
+    operator.eq: 'eq', operator.ne: 'ne',
+    torch.add: 'add', torch.sub: 'sub', torch.mul: 'mul', torch.div: 'div',
+    torch.pow: 'pow', torch.maximum: 'maximum', torch.minimum: 'minimum',
+}
+_EW_UN = {
+    operator.neg: 'neg', torch.neg: 'neg', torch.abs: 'abs', operator.abs: 'abs',
+    torch.exp: 'exp', torch.sqrt: 'sqrt', torch.sin: 'sin', torch.cos: 'cos',
+    torch.tanh: 'tanh', torch.log: 'log', torch.sign: 'sign',
+}
+_M_BIN = {'add': 'add', 'sub': 'sub', 'mul': 'mul', 'div': 'div', 'pow': 'pow'}
+_M_UN = {'neg': 'neg', 'abs': 'abs', 'exp': 'exp', 'sqrt': 'sqrt', 'sin': 'sin',
+         'cos': 'cos', 'tanh': 'tanh', 'log': 'log', 'sign': 'sign'}
+_M_INPLACE = {'add_': 'add', 'sub_': 'sub', 'mul_': 'mul', 'div_': 'div'}
+_PRED = {'lt', 'le', 'gt', 'ge', 'eq', 'ne'}
+
+
+class V:
+    """A value of the fused program."""
+    _n = 0
+
+    def __init__(self, kind, space=None, args=(), op=None, obj=None,
+                 const=None, dtype=None):
+        V._n += 1
+        self.id = V._n
+        self.kind = kind          # leaf scalar const ew gather reduce store
+        self.space = space        # index-space root, or None (scalar/const)
+        self.args = list(args)
+        self.op = op
+        self.obj = obj
+        self.const = const
+        self.dtype = dtype
+        self.stage = -1
+        self.users: List['V'] = []
+        self.mat = False          # written to a buffer by its kernel
+        self.buf: Optional[torch.Tensor] = None
+        for a in self.args:
+            a.users.append(self)
+
+
+class _Version:
+    def __init__(self, value: V):
+        self.value = value
+        self.reads = []           # (stage, is_gather)
+
+
+class Program:
+    """Symbolic execution of one forward graph."""
+
+    def __init__(self, module: _M.Module, graph: fx.Graph, prog: CompiledProgram):
+        self.m, self.g, self.prog = module, graph, prog
+        self.sa = prog.spaces
+        self.values: List[V] = []
+        self.stores: List[V] = []
+        self.version: Dict[_M.Tensor, _Version] = {}
+        self.leaf_of: Dict[_M.Tensor, V] = {}
+        self.env: Dict[fx.Node, object] = {}
+        self.dtype = None
+
+    # -- helpers --------------------------------------------------------------
+    def _new(self, *a, **k):
+        v = V(*a, **k)
+        self.values.append(v)
+        return v
+
+    def _check_dtype(self, dt):
+        if dt not in (torch.float32, torch.float64):
+            raise Unsupported(f'dtype {dt}')
+        if self.dtype is None:
+            self.dtype = dt
+        elif self.dtype != dt:
+            raise Unsupported('mixed float dtypes')
+
+    def _tensor_value(self, t: _M.Tensor) -> V:
+        if t not in self.version:
+            if t.is_replica:
+                if t.data.numel() != 1:
+                    raise Unsupported('replicated tensor with more than one element')
+                self._check_dtype(t.dtype)
+                v = self._new('scalar', None, obj=t, dtype=t.dtype)
+            else:
+                if t.data.dim() != 1:
+                    raise Unsupported('multi-column tensor')
+                self._check_dtype(t.dtype)
+                v = self._new('leaf', self.sa.root(self.sa.tensor_var[t]), obj=t,
+                              dtype=t.dtype)
+            self.leaf_of[t] = v
+            self.version[t] = _Version(v)
+        return self.version[t].value
+
+    def _arg(self, a):
+        if isinstance(a, fx.Node):
+            x = self.env[a]
+            if isinstance(x, _M.Tensor):
+                return self._tensor_value(x)
+            return x
+        if isinstance(a, bool) or not isinstance(a, (int, float)):
+            raise Unsupported(f'argument {a!r}')
+        return self._new('const', None, const=float(a))
+
+    def _space_of(self, args):
+        sp = {a.space for a in args if a.space is not None}
+        if len(sp) > 1:
+            raise Unsupported('operands from different index spaces')
+        return next(iter(sp)) if sp else None
+
+    def _ew(self, op, args):
+        args = [self._arg(a) for a in args]
+        if op == 'pow' and args[1].kind != 'const':
+            raise Unsupported('pow with a tensor exponent')
+        return self._new('ew', self._space_of(args), args=args, op=op,
+                         dtype=self.dtype)
+
+    def _store(self, target, value: V):
+        if not isinstance(target, _M.Tensor) or not target.is_partition:
+            raise Unsupported('assignment to a non-partition tensor')
+        old = self._tensor_value(target)
+        if value.kind == 'const':
+            value = self._new('ew', old.space, args=[value], op='bcast', dtype=self.dtype)
+        if value.space not in (None, old.space):
+            raise Unsupported('assignment across index spaces')
+        st = self._new('store', old.space, args=[value], obj=target, dtype=self.dtype)
+        st.old = self.version[target]
+        self.stores.append(st)
+        self.version[target] = _Version(value)
+        return st
+
+    # -- symbolic execution -------------------------------------------------
+    def run(self):
+        for node in self.g.nodes:
+            op, t = node.op, node.target
+            if node.op == 'output':
+                continue
+            if op == 'get_attr':
+                obj = resolve_attr(self.m, t)
+                if not isinstance(obj, _M.Tensor):
+                    raise Unsupported(f'attribute {t}')
+                self.env[node] = obj
+                continue
+            if node.kwargs and not (op == 'call_method' and t in _M_INPLACE
+                                    and set(node.kwargs) <= {'alpha'}
+                                    and node.kwargs.get('alpha', 1) == 1):
+                raise Unsupported(f'keyword arguments in {node.format_node()}')
+            if op == 'call_module':
+                sub = self.m.get_submodule(t)
+                x = self._arg(node.args[0])
+                if isinstance(sub, _M.Selector):
+                    if x.space != self.sa.root(self.sa.sel_vars[sub][0]):
+                        raise Unsupported('selector input space')
+                    self.env[node] = self._new(
+                        'gather', self.sa.root(self.sa.sel_vars[sub][1]), args=[x],
+                        obj=sub, dtype=self.dtype)
+                elif isinstance(sub, _M.Reducer):
+                    if sub.reduce != 'sum':
+                        raise Unsupported('non-sum reducer')
+                    if x.space != self.sa.root(self.sa.red_vars[sub][0]):
+                        raise Unsupported('reducer input space')
+                    self.env[node] = self._new(
+                        'reduce', self.sa.root(self.sa.red_vars[sub][1]), args=[x],
+                        obj=sub, dtype=self.dtype)
+                else:
+                    raise Unsupported('nested module call')
+                continue
+            if op == 'call_function':
+                if t in _EW_BIN:
+                    self.env[node] = self._ew(_EW_BIN[t], node.args)
+                elif t in _EW_UN:
+                    self.env[node] = self._ew(_EW_UN[t], node.args)
+                elif t is operator.pos:
+                    self.env[node] = self._arg(node.args[0])
+                elif t is torch.where:
+                    c, a, b = (self._arg(x) for x in node.args)
+                    self.env[node] = self._new('ew', self._space_of([c, a, b]),
+                                               args=[c, a, b], op='where',
+                                               dtype=self.dtype)
+                elif t is operator.getitem:
+                    x, idx = node.args
+                    if idx != slice(None) and idx is not Ellipsis:
+                        raise Unsupported('indexing')
+                    self.env[node] = self._arg(x)
+                elif t is operator.setitem:
+                    tgt, idx, val = node.args
+                    if idx != slice(None) and idx is not Ellipsis:
+                        raise Unsupported('partial assignment')
+                    self._store(self.env[tgt], self._arg(val))
+                    self.env[node] = None
+                else:
+                    raise Unsupported(f'function {getattr(t, "__name__", t)}')
+                continue
+            if op == 'call_method':
+                if t in _M_INPLACE:
+                    tgt = self.env[node.args[0]]
+                    val = self._ew(_M_INPLACE[t], node.args)
+                    self._store(tgt, val)
+                    self.env[node] = self._tensor_value(tgt)
+                elif t == 'copy_':
+                    tgt = self.env[node.args[0]]
+                    self._store(tgt, self._arg(node.args[1]))
+                    self.env[node] = self._tensor_value(tgt)
+                elif t in _M_BIN:
+                    self.env[node] = self._ew(_M_BIN[t], node.args)
+                elif t in _M_UN:
+                    self.env[node] = self._ew(_M_UN[t], node.args)
+                else:
+                    raise Unsupported(f'method {t}')
+                continue
+            raise Unsupported(node.op)
+        if not self.stores:
+            raise Unsupported('forward assigns nothing')
+        if self.dtype is None:
+            raise Unsupported('no float tensors')
+
+    # -- scheduling -----------------------------------------------------------
+    def schedule(self):
+        """Assign every value the stage of the kernel that evaluates it.
+
+        c(leaf) = -1 (ready before the forward); c(gather) = 0 for a leaf
+        source, else c(source) + 1 (the source must be complete); c(ew/reduce)
+        = max(0, c(args)). Edge-space values are re-evaluated inside each
+        reduce loop that uses them, so their evaluation stages are those of
+        their consumers. A store runs no earlier than its value and after
+        every evaluation that reads the stored tensor's old contents (strictly
+        after, for reads at other rows through a Selector)."""
+        sa = self.sa
+        self.edge_spaces = {sa.root(sa.red_vars[r][0]) for r in sa.red_vars}
+        for v in self.values:
+            if v.kind in ('leaf', 'scalar', 'const'):
+                v.stage = -1
+            elif v.kind == 'gather':
+                src = v.args[0]
+                v.stage = 0 if src.kind in ('leaf', 'scalar') else src.stage + 1
+            elif v.kind in ('ew', 'reduce'):
+                v.stage = max([0] + [a.stage for a in v.args])
+        for st in self.stores:
+            if st.space in self.edge_spaces:
+                raise Unsupported('assignment to an edge tensor')
+        # evaluation stages (backwards: users come later in creation order)
+        evals: Dict[int, set] = {}
+        for v in reversed(self.values):
+            if v.kind == 'store':
+                continue
+            if v.kind in ('gather', 'ew') and v.space in self.edge_spaces:
+                s = set()
+                for u in v.users:
+                    if u.kind == 'reduce':
+                        s.add(u.stage)
+                    else:
+                        s |= evals.get(u.id, {u.stage})
+                evals[v.id] = s or {v.stage}
+            else:
+                evals[v.id] = {v.stage}
+        self.evals = evals
+        # stores, in program order
+        last_store: Dict[_M.Tensor, int] = {}
+        for st in self.stores:
+            val = st.args[0]
+            s = max(0, val.stage, last_store.get(st.obj, 0))
+            leaf = self.leaf_of.get(st.obj)
+            if leaf is not None:
+                for u in leaf.users:
+                    if u is st:
+                        continue
+                    for es in evals.get(u.id, {u.stage}):
+                        s = max(s, es + (1 if u.kind == 'gather' else 0))
+            st.stage = s
+            last_store[st.obj] = s
+
+def _fmt_const(x: float, ctype: str) -> str:
+    if x != x:
+        return f'({ctype})(__builtin_nan(""))'
+    if x in (float('inf'), float('-inf')):
+        return f'({ctype})({"-" if x < 0 else ""}__builtin_inf())'
+    return f'({ctype})({float(x).hex()})'
+
+
+_CODE_HEADER = r'''
+typedef long long i64;
+#define ESR_XCD 8
+__device__ __forceinline__ i64 esr_remap(i64 b, i64 nb) {
+  const i64 q = nb / ESR_XCD, r = nb % ESR_XCD;
+  const i64 x = b % ESR_XCD, pos = b / ESR_XCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
+}
+'''
+
+
+class Kernel:
+    def __init__(self, space, stage, name):
+        self.space, self.stage, self.name = space, stage, name
+        self.params: List[tuple] = []     # (cname, ctype, tensor)
+        self.pindex: Dict[int, str] = {}
+        self.n = 0
+        self.src = ''
+
+
+class FusedEngine:
+    """Executes the fused kernels of one forward; captured as a HIP graph
+    after the first call."""
+
+    def __init__(self, module, launches, keepalive):
+        self.module = module
+        self.launches = launches          # (fn, grid, argbytes)
+        self.keepalive = keepalive
+        self.graph = None
+        self.calls = 0
+
+    def _launch_all(self, stream):
+        for fn, grid, buf in self.launches:
+            nat.call('esr_launch', fn, grid, 1, 1, 256, 1, 1, 0, stream,
+                     ctypes.addressof(buf), ctypes.sizeof(buf))
+
+    def forward(self):
+        self.calls += 1
+        if self.graph is not None:
+            nat.call('esr_graph_launch', self.graph.exec, nat.stream_ptr())
+            return None
+        self._launch_all(nat.stream_ptr())
+        if self.calls >= 2:
+            from ..numeric.solver.fused import _Graph
+            self.graph = _Graph(self._launch_all)
+        return None
+
+
+class _Lowering:
+    def __init__(self, p: Program, prog: CompiledProgram):
+        self.p, self.prog = p, prog
+        self.ctype = 'double' if p.dtype == torch.float64 else 'float'
+        self.device = prog.device
+        self.kernels: List[Kernel] = []
+
+    def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
+        a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
+        return a is not None and b is not None and a.shape == b.shape and \
+            red.easier_perm is None and np.array_equal(a, b)
+
+    def materialize(self):
+        p = self.p
+        for v in p.values:
+            if v.kind not in ('ew', 'reduce', 'gather') or v.space is None or \
+                    v.space in p.edge_spaces:
+                continue
+            need = any(u.kind == 'gather' or u.stage != v.stage or
+                       u.space not in (v.space, None) for u in v.users)
+            v.mat = need
+            if need:
+                n = self.prog.space_len[v.space]
+                v.buf = torch.empty(n, dtype=p.dtype, device=self.device)
+
+    def build(self):
+        p = self.p
+        groups: Dict[tuple, list] = {}
+        for v in p.values:
+            if v.kind == 'store' or (v.mat and v.kind != 'leaf'):
+                groups.setdefault((v.stage, v.space), []).append(v)
+        srcs = []
+        for k, (stage, space) in enumerate(sorted(groups, key=lambda x: (x[0], str(x[1])))):
+            roots = groups[(stage, space)]
+            kern = Kernel(space, stage, f'esr_fused_{k}')
+            kern.n = self.prog.space_len[space]
+            self._emit_kernel(kern, roots)
+            self.kernels.append(kern)
+            srcs.append(kern.src)
+        return _CODE_HEADER + '\n'.join(srcs)
+
+    # -- code emission --------------------------------------------------------
+    def _param(self, kern, tensor, ctype, const=True):
+        key = tensor.data_ptr()
+        if key in kern.pindex:
+            name = kern.pindex[key]
+            if not const:       # read and written by this kernel
+                k = int(name[3:])
+                n_, _, t_ = kern.params[k]
+                kern.params[k] = (n_, ctype + '*', t_)
+            return name
+        name = f'a.p{len(kern.params)}'
+        kern.params.append((name, ('const ' if const else '') + ctype + '*', tensor))
+        kern.pindex[key] = name
+        return name
+
+    def _emit_kernel(self, kern: Kernel, roots: List[V]):
+        ct = self.ctype
+        rows: List[str] = []
+        memo: Dict[int, str] = {}
+        done_groups = set()
+
+        def buf_of(v: V):
+            if v.kind == 'leaf':
+                return self._param(kern, v.obj.data, ct)
+            if v.kind == 'scalar':
+                return self._param(kern, v.obj.data, ct)
+            assert v.mat and v.buf is not None
+            return self._param(kern, v.buf, ct)
+
+        def var(stmts, expr, typ=None):
+            name = f'v{len(memo_names)}'
+            memo_names.append(name)
+            stmts.append(f'  const {typ or ct} {name} = {expr};')
+            return name
+        memo_names: List[str] = []
+
+        def ew_expr(v, a):
+            op = v.op
+            if op == 'add': return f'({a[0]} + {a[1]})'
+            if op == 'sub': return f'({a[0]} - {a[1]})'
+            if op == 'mul': return f'({a[0]} * {a[1]})'
+            if op == 'div': return f'({a[0]} / {a[1]})'
+            if op == 'neg': return f'(-{a[0]})'
+            if op == 'bcast': return a[0]
+            if op == 'abs': return f'fabs({a[0]})'
+            if op in ('exp', 'sqrt', 'sin', 'cos', 'tanh', 'log'):
+                return f'{op}({a[0]})'
+            if op == 'sign':
+                return (f'(({a[0]}) > ({ct})0 ? ({ct})1 : (({a[0]}) < ({ct})0 ? '
+                        f'({ct})-1 : ({a[0]}) * ({ct})0))')
+            if op == 'pow':
+                e = v.args[1].const
+                if e == 2: return f'({a[0]} * {a[0]})'
+                if e == 3: return f'({a[0]} * {a[0]} * {a[0]})'
+                if e == 0.5: return f'sqrt({a[0]})'
+                if e == -1: return f'(({ct})1 / {a[0]})'
+                return f'pow({a[0]}, {a[1]})'
+            if op in _PRED:
+                sym = {'lt': '<', 'le': '<=', 'gt': '>', 'ge': '>=', 'eq': '==', 'ne': '!='}[op]
+                return f'(({a[0]}) {sym} ({a[1]}))'
+            if op == 'where': return f'(({a[0]}) ? ({a[1]}) : ({a[2]}))'
+            if op == 'maximum':
+                return f'(({a[0]} != {a[0]} || {a[0]} > {a[1]}) ? {a[0]} : {a[1]})'
+            if op == 'minimum':
+                return f'(({a[0]} != {a[0]} || {a[0]} < {a[1]}) ? {a[0]} : {a[1]})'
+            raise Unsupported(op)
+
+        def is_pred(v):
+            return v.kind == 'ew' and v.op in _PRED
+
+        def emit_row(v: V) -> str:
+            if v.id in memo:
+                return memo[v.id]
+            if v.kind == 'const':
+                s = _fmt_const(v.const, ct)
+            elif v.kind == 'scalar':
+                s = var(rows, f'{buf_of(v)}[0]')
+            elif v.kind == 'leaf' or (v.mat and v.stage < kern.stage) or \
+                    (v.mat and v.space != kern.space):
+                if v.space != kern.space:
+                    raise Unsupported('row read from another space')
+                s = var(rows, f'{buf_of(v)}[r]')
+            elif v.kind == 'gather':
+                sel = v.obj
+                src = v.args[0]
+                idx = self._param(kern, sel.idx, 'int')
+                s = var(rows, f'{buf_of(src)}[{idx}[r]]')
+            elif v.kind == 'reduce':
+                emit_group(v)
+                s = memo[v.id]
+            elif v.kind == 'ew':
+                a = [emit_row(x) for x in v.args]
+                s = var(rows, ew_expr(v, a), 'bool' if is_pred(v) else None)
+            else:
+                raise Unsupported(v.kind)
+            memo[v.id] = s
+            return s
+
+        def emit_group(v: V):
+            red = v.obj
+            key = (id(red),)
+            if key in done_groups:
+                return
+            done_groups.add(key)
+            members = [u for u in self.p.values if u.kind == 'reduce' and u.obj is red
+                       and u.stage == kern.stage and u.space == kern.space
+                       and (u.id in needed)]
+            rp = self._param(kern, red.easier_rowptr, 'int')
+            perm = None if red.easier_perm is None else \
+                self._param(kern, red.easier_perm, 'int')
+            body: List[str] = []
+            emem: Dict[int, str] = {}
+
+            def emit_edge(x: V) -> str:
+                if x.id in emem:
+                    return emem[x.id]
+                if x.space != v.args[0].space:
+                    s = emit_row(x)           # row value or constant, hoisted
+                elif x.kind == 'leaf':
+                    s = var(body, f'{buf_of(x)}[e]')
+                elif x.kind == 'gather':
+                    src = x.args[0]
+                    if self._rowlocal(x.obj, red):
+                        s = emit_row(src) if src.kind != 'leaf' else \
+                            emit_row_leaf(src)
+                    else:
+                        idx = self._param(kern, x.obj.idx, 'int')
+                        s = var(body, f'{buf_of(src)}[{idx}[e]]')
+                elif x.kind == 'ew':
+                    a = [emit_edge(y) for y in x.args]
+                    s = var(body, ew_expr(x, a), 'bool' if is_pred(x) else None)
+                else:
+                    raise Unsupported(f'{x.kind} in an edge expression')
+                emem[x.id] = s
+                return s
+
+            def emit_row_leaf(src):
+                return emit_row(src)
+
+            accs = []
+            for u in members:
+                accs.append(f'acc{u.id}')
+            inner = [emit_edge(u.args[0]) for u in members]
+            for u, a in zip(members, accs):
+                body.append(f'  {a} = {a} + {inner[members.index(u)]};')
+            rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
+            rows.append(f'  for (int j = {rp}[r]; j < {rp}[r + 1]; ++j) {{')
+            rows.append(f'    const int e = {perm + "[j]" if perm else "j"};')
+            rows.extend('  ' + b for b in body)
+            rows.append('  }')
+            for u, a in zip(members, accs):
+                memo[u.id] = a
+
+        # values this kernel must compute (stop at values other kernels write)
+        needed = set()
+
+        def mark(v: V):
+            if v.id in needed:
+                return
+            if v.kind in ('leaf', 'scalar', 'const'):
+                return
+            if v.mat and v.stage != kern.stage and v.kind != 'store':
+                return
+            needed.add(v.id)
+            for a in v.args:
+                mark(a)
+        for rt in roots:
+            needed.add(rt.id)
+            for a in rt.args:
+                mark(a)
+
+        stores = []
+        for rt in roots:
+            if rt.kind == 'store':
+                val = rt.args[0]
+                if val.mat and val.stage != kern.stage:
+                    s = var(rows, f'{buf_of(val)}[r]')
+                else:
+                    s = emit_row(val)
+                stores.append(f'  {self._param(kern, rt.obj.data, ct, const=False)}[r] = {s};')
+            else:
+                s = emit_row(rt)
+                stores.append(f'  {self._param(kern, rt.buf, ct, const=False)}[r] = {s};')
+        params = ''.join(f'  {ctyp} __restrict__ p{i};\n'
+                         for i, (_, ctyp, _) in enumerate(kern.params))
+        kern.src = (
+            f'struct Args_{kern.name} {{\n{params}  i64 n;\n}};\n'
+            f'extern "C" __global__ __launch_bounds__(256) void {kern.name}'
+            f'(const Args_{kern.name} a) {{\n'
+            f'  const i64 r = esr_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;\n'
+            f'  if (r >= a.n) return;\n' + '\n'.join(rows) + '\n' +
+            '\n'.join(stores) + '\n}\n')
+
+    def pack(self, kern: Kernel) -> bytes:
+        vals = [t.data_ptr() for (_, _, t) in kern.params]
+        return struct.pack(f'{len(vals)}Qq', *vals, kern.n)
+
+
+_MODULE_CACHE: Dict[str, object] = {}
+
+
+def rtc_options(gfx: str) -> str:
+    """hipRTC options: same floating-point contract as the prebuilt kernels
+    (no FMA contraction, no packed-FP32 selection)."""
+    return (f'--offload-arch={gfx} -O3 -std=c++17 -ffp-contract=off '
+            '-Xclang -target-feature -Xclang -packed-fp32-ops')
+
+
+def _compile_source(src: str):
+    key = hashlib.sha1(src.encode()).hexdigest()
+    if key in _MODULE_CACHE:
+        return _MODULE_CACHE[key]
+    arch = ctypes.create_string_buffer(256)
+    nat.call('esr_device_arch', arch, 256)
+    gfx = arch.value.decode().split(':')[0]
+    mod = ctypes.c_void_p()
+    nat.call('esr_rtc_compile', src.encode(), b'esr_fused.hip',
+             rtc_options(gfx).encode(), ctypes.byref(mod))
+    _MODULE_CACHE[key] = mod
+    return mod
+
+
+def lower_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
+    """(program, lowering, source) of a forward; raises Unsupported."""
+    p = Program(m, g, prog)
+    p.run()
+    p.schedule()
+    low = _Lowering(p, prog)
+    low.materialize()
+    src = low.build()
+    return p, low, src
+
+
+def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
+    p, low, src = lower_module(m, g, prog)
+    mod = _compile_source(src)
+    launches = []
+    for k in low.kernels:
+        fn = ctypes.c_void_p()
+        nat.call('esr_rtc_function', mod, k.name.encode(), ctypes.byref(fn))
+        raw = low.pack(k)
+        buf = (ctypes.c_char * len(raw)).from_buffer_copy(raw)
+        grid = max(1, (k.n + 255) // 256)
+        launches.append((fn, grid, buf))
+    keep = [v.buf for v in p.values if v.buf is not None]
+    eng = FusedEngine(m, launches, keep)
+    eng.source = src
+    eng.kernels = low.kernels
+    return eng
+
+
+def try_fuse_program(prog: CompiledProgram):
+    out = {}
+    for m, g in zip(prog.modules, prog.graphs):
+        try:
+            out[m] = fuse_module(m, g, prog)
+            logger.debug(f'{m.easier_hint_name}: fused into '
+                         f'{len(out[m].kernels)} kernels')
+        except Unsupported as e:
+            logger.debug(f'{m.easier_hint_name}: node-by-node engine ({e})')
+    return out

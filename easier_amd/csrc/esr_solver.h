@@ -1,0 +1,44 @@
+// Device-resident state of the fused Krylov solvers (host and device view).
+#pragma once
+#include <cstdint>
+
+namespace esr {
+
+// Jacobi/identity-preconditioned CG (easier/numeric/solver/cg.py:74-137).
+// Scalars live on the device so an iteration needs no host round trip; the
+// host polls `done` once per captured chunk of iterations.
+struct PcgState {
+  double rz_cur;   // sum(r*z) of the current r, z (Step's rzsum)
+  double rz_next;  // sum(r*z) after the next Update (committed by the next step)
+  double pAp;
+  double alpha;
+  double rr;
+  double rz_new;
+  double rnorm;    // ||r|| after the last Step (the recursive residual)
+  double bnorm;
+  double tol;      // max(rtol*||b||, atol) if atol else rtol*||b||
+  double rtol;
+  double atol;     // <= 0: unset
+  double beta;
+  int64_t k_cur;   // `iters` of the reference loop at the current Step
+  int64_t k_next;
+  int64_t maxiter; // < 0: unset
+  int64_t done;    // 1 once the loop has exited
+  int64_t iters;   // value of `iters` at exit
+  int64_t pad0;
+  uint32_t counter[4]; // last-block-done counters (zero between launches)
+};
+
+// Restarted GMRES(m) Arnoldi cycle (easier/numeric/solver/gmres.py:214-266).
+struct GmresState {
+  double rnorm;       // true residual norm of the cycle's start
+  double bnorm;
+  double H[33 * 32];  // (m+1) x m, row-major with row stride 32 (m <= 32)
+  int64_t broken;     // Arnoldi breakdown (h < 1e-15) seen at step jend
+  int64_t jend;       // last Arnoldi step of the cycle
+  uint32_t counter[4];
+};
+
+constexpr int kGmresMaxRestart = 32;
+
+} // namespace esr

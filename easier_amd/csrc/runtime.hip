@@ -92,6 +92,43 @@ ESR_API int esr_rtc_compile(const char *source, const char *name,
   ESR_GUARD_END
 }
 
+// Compile without loading (no device needed): returns ESR_ERR_RTC with the
+// compiler log on failure, and the code object size in *code_bytes.
+ESR_API int esr_rtc_check(const char *source, const char *options, int64_t *code_bytes) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(source && code_bytes, "esr_rtc_check: null argument");
+  hiprtcProgram prog;
+  hiprtcResult rc = hiprtcCreateProgram(&prog, source, "esr_check.hip", 0, nullptr, nullptr);
+  if (rc != HIPRTC_SUCCESS) {
+    esr::set_error("hiprtcCreateProgram: %s", hiprtcGetErrorString(rc));
+    return ESR_ERR_RTC;
+  }
+  std::vector<std::string> opts;
+  if (options) {
+    std::istringstream ss(options);
+    std::string tok;
+    while (ss >> tok) opts.push_back(tok);
+  }
+  std::vector<const char *> copts;
+  for (auto &o : opts) copts.push_back(o.c_str());
+  rc = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls + 1, '\0');
+    hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    esr::set_error("hipRTC compile failed: %s\n%s", hiprtcGetErrorString(rc), log.c_str());
+    return ESR_ERR_RTC;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  hiprtcDestroyProgram(&prog);
+  *code_bytes = (int64_t)cs;
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
 ESR_API int esr_rtc_function(void *module, const char *kernel_name,
                              void **fn_out) {
   ESR_REQUIRE(module && kernel_name && fn_out, "esr_rtc_function: null arg");

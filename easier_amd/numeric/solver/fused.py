@@ -1,9 +1,309 @@
-"""Fused device solvers (filled in below)."""
+"""Fused device iterations for CG and GMRES on a compiled Linsys.
+
+After esr.compile(..., 'hip'), CG/GMRES check whether their operator is a
+plain Linsys (reducer(selector(x) * Ae) + x * Av) whose Reducer fixes its edge
+order (so the edge list is a row-sorted CSR) and whose preconditioner is the
+identity or preconditioner.Jacobi. If so, solve() runs:
+
+CG: one init kernel, then chunks of iterations, each iteration three kernels
+    (SpMV + p.Ap; x/r/z update + r.r, r.z; loop test + p update) with every
+    scalar (alpha, beta, rz, ||r||, iters, done) on the device. A chunk is
+    captured once as a HIP graph and replayed; the host reads the state once
+    per chunk. Iterations after the exit test are no-ops, so `iters`,
+    `residual` and x are those of cg.py:106-137.
+GMRES: per restart cycle one residual kernel, the whole Arnoldi process as one
+    captured graph (SpMV, one kernel per MGS projection, norm), one host read of
+    H, the small SVD least-squares solve on the host, one update kernel.
+"""
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ...core import module as esr
+from ...core import ops
+from ...core.utils import logger
+from ..linsys import Linsys
+from ..preconditioner.jacobi import Jacobi
+
+_c_i32, _c_i64, _c_dbl, _c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+_P_i64 = ctypes.POINTER(ctypes.c_int64)
+_P_dbl = ctypes.POINTER(ctypes.c_double)
+
+nat.register_signatures({
+    'esr_pcg_workspace': (_c_i32, [_c_i64, _P_i64, _P_i64]),
+    'esr_pcg_init': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_dbl, _c_dbl, _c_i64, _c_vp]),
+    'esr_pcg_steps': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i64, _c_vp]),
+    'esr_pcg_read': (_c_i32, [_c_vp, _P_dbl, _P_i64, _c_vp]),
+    'esr_gmres_workspace': (_c_i32, [_c_i64, _c_i64, _P_i64, _P_i64]),
+    'esr_gmres_begin': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_gmres_residual': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 9 + [_c_vp]),
+    'esr_gmres_cycle': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 12 + [_c_vp]),
+    'esr_gmres_update_x': (_c_i32, [_c_i32, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_gmres_read': (_c_i32, [_c_vp, _c_i64, _P_dbl, _P_i64, _P_dbl, _c_vp]),
+})
+
+
+class LinsysCSR:
+    """The device CSR view of a compiled Linsys: rowptr, col (the Selector's
+    renumbered index in row-sorted edge order), Ae, Av."""
+
+    def __init__(self, A: Linsys):
+        self.rowptr = A.reducer.easier_rowptr
+        self.col = A.selector.idx
+        self.Ae = A.Ae.data
+        self.Av = A.Av.data
+        self.n = A.reducer.n
+        self.dtype = self.Av.dtype
+
+    @staticmethod
+    def applicable(A, x: esr.Tensor, prog) -> Optional[str]:
+        """None if the fused path applies, else the reason it does not."""
+        if type(A) is not Linsys:
+            return 'operator is not a Linsys'
+        sel, red = A.selector, A.reducer
+        if not isinstance(sel, esr.Selector) or not isinstance(red, esr.Reducer):
+            return 'Linsys without an esr Selector/Reducer'
+        if red.reduce != 'sum':
+            return 'Reducer is not a sum'
+        if red.easier_perm is not None:
+            return 'the Reducer does not fix the edge order'
+        for t in (A.Av, A.Ae, x):
+            if not isinstance(t, esr.Tensor) or not t.is_partition:
+                return 'Av/Ae/x are not partition tensors'
+        if A.Av.data.dim() != 1 or A.Ae.data.dim() != 1 or x.data.dim() != 1:
+            return 'only vectors (one value per vertex/edge) are fused'
+        dts = {A.Av.dtype, A.Ae.dtype, x.dtype}
+        if len(dts) != 1 or A.Av.dtype not in (torch.float32, torch.float64):
+            return 'mixed or non-float dtypes'
+        if A.Ae.data.shape[0] != sel.idx.shape[0] or red.n != x.data.shape[0]:
+            return 'shape mismatch'
+        sa = prog.spaces
+        sv = sa.root(sa.sel_vars[sel][0])
+        rv = sa.root(sa.red_vars[red][1])
+        xv = sa.root(sa.tensor_var[x])
+        if not (sv == rv == xv):
+            return 'selector source, reducer target and x are different spaces'
+        return None
+
+
+def _precond_diag(M, x: esr.Tensor, prog):
+    """(ok, diag tensor or None) for identity / Jacobi preconditioners."""
+    if M is None or getattr(M, '__name__', '') == '_identity':
+        return True, None
+    if type(M) is Jacobi and isinstance(M.diag, esr.Tensor) and M.diag.is_partition:
+        sa = prog.spaces
+        if sa.root(sa.tensor_var[M.diag]) == sa.root(sa.tensor_var[x]) and \
+                M.diag.dtype == x.dtype and M.diag.data.dim() == 1:
+            return True, M.diag.data
+    return False, None
+
+
+def _capture(launch):
+    """Capture `launch(stream_ptr)` into a HIP graph on a side stream."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    exec_ = ctypes.c_void_p()
+    with torch.cuda.stream(side):
+        sp = side.cuda_stream
+        nat.call('esr_graph_begin', sp)
+        try:
+            launch(sp)
+        finally:
+            rc = nat.lib().esr_graph_end(sp, ctypes.byref(exec_))
+        nat.check(rc, 'esr_graph_end')
+    torch.cuda.current_stream().wait_stream(side)
+    return exec_
+
+
+class _Graph:
+    def __init__(self, launch):
+        self.exec = _capture(launch)
+
+    def replay(self):
+        nat.call('esr_graph_launch', self.exec, nat.stream_ptr())
+
+    def __del__(self):
+        try:
+            if self.exec:
+                nat.lib().esr_graph_destroy(self.exec)
+        except Exception:
+            pass
+
+
+class FusedCG:
+    CHUNK = 32
+
+    def __init__(self, cg, prog):
+        self.cg = cg
+        self.op = LinsysCSR(cg.A)
+        self.diag = _precond_diag(cg.M, cg.x, prog)[1]
+        dev = cg.x.device
+        n = self.op.n
+        sb, wb = ctypes.c_int64(), ctypes.c_int64()
+        nat.call('esr_pcg_workspace', n, ctypes.byref(sb), ctypes.byref(wb))
+        self.state = torch.empty(sb.value, dtype=torch.uint8, device=dev)
+        self.ws = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+        self.Ap = torch.empty(n, dtype=self.op.dtype, device=dev)
+        self.dt = nat.dtype_code(self.op.dtype)
+        self.graph = None
+        self.graph_chunk = 0
+
+    def _arrays(self):
+        c = self.cg
+        return [self.op.rowptr.data_ptr(), self.op.col.data_ptr(),
+                self.op.Ae.data_ptr(), self.op.Av.data_ptr(),
+                None if self.diag is None else self.diag.data_ptr()]
+
+    def steps(self, count, stream):
+        c = self.cg
+        nat.call('esr_pcg_steps', self.dt, self.op.n, *self._arrays(),
+                 c.x.data.data_ptr(), c.r.data.data_ptr(), c.z.data.data_ptr(),
+                 c.p.data.data_ptr(), self.Ap.data_ptr(), self.state.data_ptr(),
+                 self.ws.data_ptr(), count, stream)
+
+    def read(self):
+        sc = (ctypes.c_double * 6)()
+        it = (ctypes.c_int64 * 3)()
+        nat.call('esr_pcg_read', self.state.data_ptr(), sc, it, nat.stream_ptr())
+        return dict(rnorm=sc[0], bnorm=sc[1], rz=sc[2], tol=sc[3], iters=it[0],
+                    done=it[1], k=it[2])
+
+    def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None,
+              chunk=None):
+        c = self.cg
+        name = c.__class__.__name__
+        nat.call('esr_pcg_init', self.dt, self.op.n, *self._arrays(),
+                 c.b.data.data_ptr(), c.x.data.data_ptr(), c.r.data.data_ptr(),
+                 c.z.data.data_ptr(), c.p.data.data_ptr(), self.state.data_ptr(),
+                 self.ws.data_ptr(), float(rtol), float(atol) if atol else 0.0,
+                 -1 if maxiter is None else int(maxiter), nat.stream_ptr())
+        chunk = chunk or (debug_iter if debug_iter else self.CHUNK)
+        if self.graph is None or self.graph_chunk != chunk:
+            self.graph = _Graph(lambda sp: self.steps(chunk, sp))
+            self.graph_chunk = chunk
+        while True:
+            self.graph.replay()
+            s = self.read()
+            if debug_iter is not None:
+                logger.info(f"{name} residual {s['rnorm']} at the "
+                            f"{s['k']}-th iteration")
+            if s['done']:
+                break
+        ops.fill_(c.rnorm.data, s['rnorm'])
+        ops.fill_(c.bnorm.data, s['bnorm'])
+        ops.fill_(c.rzsum.data, s['rz'])
+        logger.info(f"{name} solver completed with residual {s['rnorm']}"
+                    f" at the {s['iters']}-th iteration")
+        return {'residual': s['rnorm'], 'iters': s['iters']}
 
 
 def make_fused_cg(cg, prog):
-    return None
+    why = LinsysCSR.applicable(cg.A, cg.x, prog)
+    ok, _ = _precond_diag(cg.M, cg.x, prog)
+    if why is None and not ok:
+        why = 'preconditioner is neither identity nor Jacobi'
+    if why is None and (cg.b.dtype != cg.x.dtype or cg.b.data.shape != cg.x.data.shape):
+        why = 'b and x differ in dtype or shape'
+    if why is not None:
+        logger.info(f'CG: generic compiled path ({why})')
+        return None
+    return FusedCG(cg, prog)
 
 
-def make_fused_gmres(gmres, prog):
-    return None
+class FusedGMRES:
+    def __init__(self, gm, prog):
+        self.gm = gm
+        self.op = LinsysCSR(gm.A)
+        self.diag = _precond_diag(gm.M, gm.x, prog)[1]
+        dev = gm.x.device
+        n, m = self.op.n, gm.restart
+        sb, wb = ctypes.c_int64(), ctypes.c_int64()
+        nat.call('esr_gmres_workspace', n, m, ctypes.byref(sb), ctypes.byref(wb))
+        self.state = torch.empty(sb.value, dtype=torch.uint8, device=dev)
+        self.ws = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+        self.V = torch.empty((m, n), dtype=self.op.dtype, device=dev)
+        self.w = [torch.empty(n, dtype=self.op.dtype, device=dev) for _ in range(2)]
+        self.y = torch.zeros(m, dtype=self.op.dtype, device=dev)
+        self.dt = nat.dtype_code(self.op.dtype)
+        self.graph = None
+
+    def _csr(self):
+        return [self.op.rowptr.data_ptr(), self.op.col.data_ptr(),
+                self.op.Ae.data_ptr(), self.op.Av.data_ptr()]
+
+    def _cycle(self, sp):
+        g = self.gm
+        nat.call('esr_gmres_cycle', self.dt, self.op.n, g.restart, *self._csr(),
+                 None if self.diag is None else self.diag.data_ptr(),
+                 g.r.data.data_ptr(), self.V.data_ptr(), self.w[0].data_ptr(),
+                 self.w[1].data_ptr(), self.state.data_ptr(), self.ws.data_ptr(), sp)
+
+    def _read(self, with_H):
+        m = self.gm.restart
+        sc = (ctypes.c_double * 2)()
+        it = (ctypes.c_int64 * 2)()
+        H = np.zeros((m + 1) * m) if with_H else None
+        nat.call('esr_gmres_read', self.state.data_ptr(), m, sc, it,
+                 H.ctypes.data if with_H else None, nat.stream_ptr())
+        return sc[0], sc[1], it[0], it[1], (H.reshape(m + 1, m) if with_H else None)
+
+    def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None):
+        from .gmres import lstsq_svd
+        g = self.gm
+        name = g.__class__.__name__
+        n, m = self.op.n, g.restart
+        if self.graph is None:
+            self.graph = _Graph(self._cycle)
+        nat.call('esr_gmres_begin', self.dt, n, g.b.data.data_ptr(),
+                 self.state.data_ptr(), self.ws.data_ptr(), nat.stream_ptr())
+        _, bnorm, _, _, _ = self._read(False)
+        rtol = rtol * bnorm
+        tol = max(rtol, atol) if atol else rtol
+        iters = 0
+        B = g.easier_host_B
+        while True:
+            nat.call('esr_gmres_residual', self.dt, n, *self._csr(),
+                     g.b.data.data_ptr(), g.x.data.data_ptr(), g.r.data.data_ptr(),
+                     self.state.data_ptr(), self.ws.data_ptr(), nat.stream_ptr())
+            rnorm, _, _, _, _ = self._read(False)
+            if debug_iter is not None and iters % debug_iter == 0:
+                logger.info(f"{name} residual {rnorm} at the {iters}-th iteration")
+            if (rnorm == rnorm and rnorm <= tol) or \
+                    (maxiter is not None and iters >= maxiter):
+                break
+            if rnorm != rnorm and maxiter is None:
+                logger.warning(f'{name}: residual is NaN, stopping')
+                break
+            iters += 1
+            B[0, 0] = rnorm
+            self.graph.replay()
+            _, _, jend, broken, H = self._read(True)
+            g.easier_host_H[:] = H
+            y = lstsq_svd(H[:jend + 2], B[:jend + 2])
+            ops.copy_(self.y, torch.from_numpy(y[:, 0].copy()).to(self.y.dtype)
+                      .to(self.y.device))
+            nat.call('esr_gmres_update_x', self.dt, n, jend + 1, self.V.data_ptr(),
+                     self.y.data_ptr(),
+                     None if self.diag is None else self.diag.data_ptr(),
+                     g.x.data.data_ptr(), nat.stream_ptr())
+        ops.fill_(g.rnorm.data, rnorm)
+        ops.fill_(g.bnorm.data, bnorm)
+        logger.info(f"{name} solver completed with residual {rnorm}"
+                    f" at the {iters}-th iteration")
+        return {'residual': rnorm, 'iters': iters}
+
+
+def make_fused_gmres(gm, prog):
+    why = LinsysCSR.applicable(gm.A, gm.x, prog)
+    ok, _ = _precond_diag(gm.M, gm.x, prog)
+    if why is None and not ok:
+        why = 'preconditioner is neither identity nor Jacobi'
+    if why is None and gm.restart > 32:
+        why = 'restart > 32'
+    if why is not None:
+        logger.info(f'GMRES: generic compiled path ({why})')
+        return None
+    return FusedGMRES(gm, prog)

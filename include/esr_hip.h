@@ -168,6 +168,8 @@ int esr_gemv_rows(int dtype, const void *A, int64_t n, int64_t kdim,
  * fused kernels compiled once per esr.compile() (hipRTC, gfx950).            */
 int esr_rtc_compile(const char *source, const char *name, const char *options,
                     void **module_out);
+int esr_rtc_check(const char *source, const char *options,
+                  int64_t *code_bytes); /* compile only, no device needed */
 int esr_rtc_function(void *module, const char *kernel_name, void **fn_out);
 int esr_rtc_unload(void *module);
 int esr_launch(void *fn, uint32_t gx, uint32_t gy, uint32_t gz, uint32_t bx,
@@ -215,6 +217,73 @@ int esr_rcm(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
 /* Largest |row(e) - col(e)| over the pairs: the bandwidth RCM minimises.     */
 int esr_bandwidth(const int64_t *a, const int64_t *b, int64_t m,
                   int64_t *bw);
+
+/* ---- a12: P1 finite elements (host) ----------------------------------------
+ * The Poisson operator in the Linsys layout the reference's Poisson tutorial
+ * feeds to CG/GMRES (tutorial/poisson/poisson_main.py:28-44: Reducer(row),
+ * Selector(col), Ae per directed edge, Av per vertex), here assembled from P1
+ * elements (the reference assembles a cell-centred finite-volume operator,
+ * tutorial/poisson/assemble_poisson.py:160-230). esr_p1_nnz gives the number
+ * of directed vertex edges; esr_p1_assemble fills row/col (sorted by row,
+ * then col), Ae, Av, b = M f (consistent mass matrix; NULL f or b skips it)
+ * and the boundary mask, with zero Dirichlet rows/columns. kappa (per cell)
+ * may be NULL (kappa = 1).                                                   */
+int esr_p1_nnz(int64_t np_, int64_t nc, const int64_t *cells, int64_t *nnz);
+int esr_p1_assemble(int64_t np_, int64_t nc, const int64_t *cells,
+                    const double *points, const double *kappa, const double *f,
+                    int64_t nnz, int64_t *row, int64_t *col, double *Ae,
+                    double *Av, double *b, uint8_t *boundary);
+
+/* ---- a3/a5: fused Jacobi/identity-preconditioned CG ------------------------
+ * easier/numeric/solver/cg.py:12-137 on a Linsys (linsys.py:23-24) given as
+ * a row-sorted CSR (rowptr, col = Selector index, Ae, Av). diag (nullable) is
+ * the Jacobi diagonal: M(r) = r / diag. State lives on the device
+ * (esr_pcg_workspace sizes it): esr_pcg_init runs Init and fixes
+ * tol = max(rtol*||b||, atol) if atol > 0 else rtol*||b|| and maxiter (< 0:
+ * none); esr_pcg_steps enqueues `count` iterations of Step + loop test +
+ * Update, each a no-op once the loop has exited (safe to capture in a graph);
+ * esr_pcg_read copies scalars {rnorm, bnorm, rz, tol, alpha, beta} and ints
+ * {iters, done, k} to the host. dtype: ESR_F64 or ESR_F32 (dots always
+ * accumulate in double).                                                     */
+int esr_pcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes);
+int esr_pcg_init(int dtype, int64_t n, const int32_t *rowptr,
+                 const int32_t *col, const void *Ae, const void *Av,
+                 const void *diag, const void *b, void *x, void *r, void *z,
+                 void *p, void *state, void *ws, double rtol, double atol,
+                 int64_t maxiter, void *stream);
+int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr,
+                  const int32_t *col, const void *Ae, const void *Av,
+                  const void *diag, void *x, void *r, void *z, void *p,
+                  void *Ap, void *state, void *ws, int64_t count,
+                  void *stream);
+int esr_pcg_read(const void *state, double *scalars, int64_t *ints,
+                 void *stream);
+
+/* ---- a6: fused GMRES(m) Arnoldi cycle ---------------------------------------
+ * easier/numeric/solver/gmres.py:12-266 (m <= 32). esr_gmres_begin: ||b||;
+ * esr_gmres_residual: r = b - A x, ||r|| (UpdateRnorm); esr_gmres_cycle:
+ * V_0 = r/||r||, then for j < m: w = A(M(V_j)), modified Gram-Schmidt against
+ * V_0..V_j into H[:, j], H[j+1, j] = ||w||, V_{j+1} = w / H[j+1, j], stopping
+ * (later kernels no-op) at H[j+1, j] < 1e-15; V is m x n (one basis vector per
+ * row), w0/w1 scratch n-vectors. esr_gmres_read: scalars {rnorm, bnorm}, ints
+ * {jend, broken}, H ((m+1) x m row-major). esr_gmres_update_x:
+ * x += M(sum_{i<jcount} V_i y_i) (UpdateX).                                  */
+int esr_gmres_workspace(int64_t n, int64_t restart, int64_t *state_bytes,
+                        int64_t *ws_bytes);
+int esr_gmres_begin(int dtype, int64_t n, const void *b, void *state,
+                    void *ws, void *stream);
+int esr_gmres_residual(int dtype, int64_t n, const int32_t *rowptr,
+                       const int32_t *col, const void *Ae, const void *Av,
+                       const void *b, const void *x, void *r, void *state,
+                       void *ws, void *stream);
+int esr_gmres_cycle(int dtype, int64_t n, int64_t restart,
+                    const int32_t *rowptr, const int32_t *col, const void *Ae,
+                    const void *Av, const void *diag, const void *r, void *V,
+                    void *w0, void *w1, void *state, void *ws, void *stream);
+int esr_gmres_update_x(int dtype, int64_t n, int64_t jcount, const void *V,
+                       const void *y, const void *diag, void *x, void *stream);
+int esr_gmres_read(const void *state, int64_t restart, double *scalars,
+                   int64_t *ints, double *H, void *stream);
 
 #ifdef __cplusplus
 }

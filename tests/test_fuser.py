@@ -1,0 +1,101 @@
+"""The fusion compiler on the host: which forwards it takes, how it stages
+them, and that the generated kernels compile for gfx950 (hipRTC, no device).
+"""
+import ctypes
+
+import pytest
+import torch
+
+import easier_amd as esr
+from easier_amd import _native as nat
+from easier_amd.core import fuser
+from easier_amd.core.compiler import compile_program
+from easier_amd.numeric import Linsys
+from easier_amd.numeric.solver import CG
+from easier_amd.workloads import mesh as wmesh
+from easier_amd.workloads import poisson as wp
+from easier_amd.workloads import swe as wswe
+from oracle import swe as oswe
+
+
+def _rtc_ok(src):
+    nb = ctypes.c_int64()
+    rc = nat.lib().esr_rtc_check(src.encode(), fuser.rtc_options('gfx950').encode(),
+                                 ctypes.byref(nb))
+    assert rc == 0, nat.lib().esr_last_error().decode()
+    assert nb.value > 0
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('kind', ['tri2', 'tri4'])
+def test_swe_step_fuses_into_four_kernels(dtype, kind):
+    mesh = getattr(wmesh, kind)(5)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81, dtype=dtype)
+    prog = compile_program([eqn], 'rcm', torch.device('cpu'))
+    p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    # one kernel per RK4 stage: the stage inputs are gathered by the next one
+    assert [k.stage for k in low.kernels] == [0, 1, 2, 3]
+    # stage inputs U1..U3 and stage deltas d1..d3 (3 fields each) are written
+    assert sum(v.mat for v in p.values) == 18
+    assert [st.stage for st in p.stores] == [3, 3, 3]
+    ctype = 'double' if dtype == torch.float64 else 'float'
+    assert f'const {ctype}' in src
+    _rtc_ok(src)
+
+
+def test_stores_wait_for_gathers_of_old_contents():
+    class M(esr.Module):
+        def __init__(self):
+            super().__init__()
+            idx = torch.tensor([1, 0, 2, 2, 0, 1])
+            self.red = esr.Reducer(torch.tensor([0, 0, 1, 1, 2, 2]), 3)
+            self.sel = esr.Selector(idx)
+            self.u = esr.Tensor(torch.rand(3, dtype=torch.double), mode='partition')
+            self.w = esr.Tensor(torch.rand(6, dtype=torch.double), mode='partition')
+
+        def forward(self):
+            s = self.red(self.sel(self.u) * self.w)
+            self.u[:] = self.u + s          # reads u at other rows first
+            t = self.red(self.sel(self.u) * self.w)
+            self.u[:] = self.u * 2 + t
+
+    m = M()
+    prog = compile_program([m], 'rcm', torch.device('cpu'))
+    p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    assert [st.stage for st in p.stores] == [1, 1]
+    assert len(low.kernels) == 2
+    _rtc_ok(src)
+
+
+def test_unsupported_forwards_keep_the_node_engine():
+    mesh, sys_ = wp.config1_system(6)
+    eqn = wp.PoissonP1(sys_)
+    sol = CG(eqn.A, eqn.b, eqn.x, eqn.M)
+    prog = compile_program([eqn, sol], 'rcm', torch.device('cpu'))
+    for m, g in zip(prog.modules, prog.graphs):
+        name = m.easier_hint_name
+        if name.endswith('.step') or name.endswith('.init') or name.endswith('.update'):
+            with pytest.raises(fuser.Unsupported):
+                fuser.lower_module(m, g, prog)      # aggregators
+
+
+def test_linsys_apply_fuses():
+    mesh, sys_ = wp.config1_system(6)
+
+    class Apply(esr.Module):
+        def __init__(self):
+            super().__init__()
+            self.p = wp.PoissonP1(sys_)
+            self.y = esr.Tensor(esr.zeros((sys_['Av'].shape[0],), dtype=torch.double),
+                                mode='partition')
+
+        def forward(self):
+            self.y[:] = self.p.A(self.p.b)
+
+    m = Apply()
+    prog = compile_program([m], 'rcm', torch.device('cpu'))
+    idx = [i for i, mm in enumerate(prog.modules) if mm is m][0]
+    p, low, src = fuser.lower_module(prog.modules[idx], prog.graphs[idx], prog)
+    assert len(low.kernels) == 1
+    _rtc_ok(src)

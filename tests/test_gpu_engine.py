@@ -132,17 +132,19 @@ def test_swe_geometry_programs(gpu, gen):
     torch.testing.assert_close(geo['h'].cpu(), ref['h'], rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize('fuse', [True, False])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 @pytest.mark.parametrize('gen,g', [('tri4', 1.0), ('tri2', 9.81)])
-def test_swe_steps_match_oracle(gpu, dtype, gen, g):
-    print(f'swe {gen} g={g} {dtype}', flush=True)
+def test_swe_steps_match_oracle(gpu, dtype, gen, g, fuse):
+    print(f'swe {gen} g={g} {dtype} fuse={fuse}', flush=True)
     mesh = getattr(wmesh, gen)(10)
     geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
     dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], g)
     eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=g, dtype=dtype)
-    esr.compile([eqn], 'hip')
+    esr.compile([eqn], 'hip', fuse=fuse)
+    assert (type(eqn.easier_engine).__name__ == 'FusedEngine') == fuse
     ref = oswe.SWE(mesh, geo, dt=dt, g=g, dtype=dtype)
-    for _ in range(20):
+    for _ in range(20):           # from the 2nd call on: graph replays
         eqn()
         ref()
     tol = 1e-12 if dtype == torch.float64 else 1e-5
@@ -150,3 +152,27 @@ def test_swe_steps_match_oracle(gpu, dtype, gen, g):
         got, want = getattr(eqn, k).collect(), getattr(ref, k)
         scale = float(want.abs().max())
         assert float((got - want).abs().max()) <= tol * scale, k
+        if dtype == torch.float64:
+            # same operations in the same order, no FMA contraction
+            assert torch.equal(got, want), k
+
+
+def test_fused_cg_graph_replays_are_deterministic(gpu):
+    eqn = Poisson5(40, seed=3)
+    sol = CG(eqn.A, eqn.b, eqn.x, Jacobi(eqn.Ac))
+    esr.compile([eqn, sol], 'hip')
+    assert sol.easier_fused is not None
+    info1 = sol.solve(rtol=1e-9, maxiter=5000)
+    x1 = eqn.x.collect()
+    ops_zero = torch.zeros_like(eqn.x.data)
+    eqn.x.data.copy_(ops_zero)
+    info2 = sol.solve(rtol=1e-9, maxiter=5000)
+    assert info1 == info2
+    assert torch.equal(x1, eqn.x.collect())
+    # the generic compiled modules agree with the fused iteration
+    eqn.x.data.copy_(ops_zero)
+    fused, sol.easier_fused = sol.easier_fused, None
+    info3 = sol.solve(rtol=1e-9, maxiter=5000)
+    sol.easier_fused = fused
+    assert abs(info3['iters'] - info1['iters']) <= 1
+    torch.testing.assert_close(eqn.x.collect(), x1, rtol=1e-8, atol=1e-10)
