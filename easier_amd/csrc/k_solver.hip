@@ -347,6 +347,52 @@ __global__ __launch_bounds__(kB) void k_gm_update_x(int64_t n, int jc, const T *
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision iterative refinement (FP64 outer loop around FP32 CG)
+// ---------------------------------------------------------------------------
+struct IrState {
+  double rnorm;
+  double bnorm;
+  uint32_t counter[2];
+};
+
+// r = b - A x in float64; writes r rounded to float32 and ||r|| (float64);
+// with first != 0 also ||b||.
+__global__ __launch_bounds__(kB) void k_ir_residual(
+    int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const double *__restrict__ Ae, const double *__restrict__ Av,
+    const double *__restrict__ bvec, const double *__restrict__ x, float *__restrict__ r32,
+    IrState *st, double *part, int first) {
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  double acc[2] = {0.0, 0.0};
+  for (int k = 0; k < kRowsPerThread; ++k) {
+    const int64_t i = b * kTile + k * kB + threadIdx.x;
+    if (i >= n) break;
+    const double ri =
+        bvec[i] - spmv_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
+    r32[i] = (float)ri;
+    acc[0] += ri * ri;
+    acc[1] += bvec[i] * bvec[i];
+  }
+  double tot[2];
+  if (grid_sum_last<2>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
+    st->rnorm = sqrt(tot[0]);
+    if (first) st->bnorm = sqrt(tot[1]);
+  }
+}
+
+// x += (double) d; d = 0 (the next inner solve starts from zero).
+__global__ __launch_bounds__(kB) void k_ir_update(int64_t n, double *__restrict__ x,
+                                                 float *__restrict__ d) {
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  for (int k = 0; k < kRowsPerThread; ++k) {
+    const int64_t i = b * kTile + k * kB + threadIdx.x;
+    if (i >= n) break;
+    x[i] = x[i] + (double)d[i];
+    d[i] = 0.0f;
+  }
+}
+
 inline unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n, kTile); }
 
 } // namespace
@@ -546,5 +592,51 @@ ESR_API int esr_gmres_read(const void *state, int64_t restart, double *scalars, 
   if (H)
     for (int64_t i = 0; i <= restart; ++i)
       for (int64_t j = 0; j < restart; ++j) H[i * restart + j] = h.H[i * kGmresMaxRestart + j];
+  return ESR_OK;
+}
+
+ESR_API int esr_ir_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
+  ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_ir_workspace: bad args");
+  *state_bytes = (int64_t)sizeof(IrState);
+  *ws_bytes = 2 * (int64_t)grid_of(n) * (int64_t)sizeof(double);
+  return ESR_OK;
+}
+
+ESR_API int esr_ir_residual(int64_t n, const int32_t *rowptr, const int32_t *col, const double *Ae,
+                            const double *Av, const double *b, const double *x, float *r32,
+                            void *state, void *ws, int first, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && rowptr && col && Ae && Av && b && x && r32 && state && ws,
+              "esr_ir_residual: null argument");
+  hipStream_t s = as_stream(stream);
+  if (first) {
+    IrState h{};
+    ESR_HIP(hipMemcpyAsync(state, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    ESR_HIP(hipStreamSynchronize(s));
+  }
+  k_ir_residual<<<grid_of(n), kB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r32, (IrState *)state,
+                                          (double *)ws, first);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_ir_update(int64_t n, double *x, float *d, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && x && d, "esr_ir_update: null argument");
+  k_ir_update<<<grid_of(n), kB, 0, as_stream(stream)>>>(n, x, d);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_ir_read(const void *state, double *scalars, void *stream) {
+  ESR_REQUIRE(state && scalars, "esr_ir_read: null argument");
+  IrState h;
+  hipStream_t s = as_stream(stream);
+  ESR_HIP(hipMemcpyAsync(&h, state, sizeof(h), hipMemcpyDeviceToHost, s));
+  ESR_HIP(hipStreamSynchronize(s));
+  scalars[0] = h.rnorm;
+  scalars[1] = h.bnorm;
   return ESR_OK;
 }

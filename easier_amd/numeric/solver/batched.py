@@ -1,0 +1,183 @@
+"""Batched Jacobi-PCG: many independent Poisson systems (uneven meshes, or
+many right-hand sides of one mesh) solved together.
+
+No reference counterpart (the reference's CG solves one system). Each
+instance follows cg.py:106-137 exactly (its own tol, iters, residual);
+instances are concatenated into one CSR, processed by the same three kernels
+per iteration (SpMV + p.Ap; x/r/z + r.r, r.z; test + p), with per-instance
+device scalars and convergence masks: a converged instance's workgroups return
+at once. The iteration loop is one captured HIP graph replayed until the
+device counter of active instances reaches zero. The tile height (256*R rows)
+is picked by a small autotuner that times a few iterations of each candidate
+on the actual batch.
+"""
+import ctypes
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ...core import ops
+from .fused import _Graph
+
+_c_i32, _c_i64, _c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+nat.register_signatures({
+    'esr_bcg_state_bytes': (_c_i32, [ctypes.POINTER(_c_i64)]),
+    'esr_bcg_init': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 17 + [_c_vp]),
+    'esr_bcg_steps': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 18 + [_c_i64, _c_vp]),
+})
+
+_STATE_DTYPE = np.dtype([
+    ('rz_cur', 'f8'), ('rz_next', 'f8'), ('pAp', 'f8'), ('alpha', 'f8'),
+    ('rr', 'f8'), ('rz_new', 'f8'), ('rnorm', 'f8'), ('bnorm', 'f8'),
+    ('tol', 'f8'), ('rtol', 'f8'), ('atol', 'f8'), ('pad', 'f8'),
+    ('k_cur', 'i8'), ('k_next', 'i8'), ('maxiter', 'i8'), ('done', 'i8'),
+    ('iters', 'i8'), ('pad2', 'i8'), ('counter', 'u4', (4,))])
+_CANDIDATES = (1, 2, 4, 8)
+
+
+class BatchedCG:
+    """systems: list of dicts with host arrays row, col (int64, CSR order:
+    sorted by row), Ae, Av, b (float64), as workloads.poisson.p1_system
+    returns. Solutions land in `x` (one float64 array per instance via
+    solutions())."""
+
+    def __init__(self, systems: List[Dict[str, np.ndarray]], jacobi=True,
+                 device=None):
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        sb = ctypes.c_int64()
+        nat.call('esr_bcg_state_bytes', ctypes.byref(sb))
+        assert sb.value == _STATE_DTYPE.itemsize, (sb.value, _STATE_DTYPE.itemsize)
+        self.K = len(systems)
+        self.sizes = np.array([s['Av'].shape[0] for s in systems], np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+        N = int(self.offsets[-1])
+        nnz = np.array([s['row'].shape[0] for s in systems], np.int64)
+        if N >= 2 ** 31 or nnz.sum() >= 2 ** 31:
+            raise ValueError('batch too large for int32 indices')
+        rowptr = np.zeros(N + 1, np.int64)
+        cols, Ae, Av, b = [], [], [], []
+        for k, s in enumerate(systems):
+            o = self.offsets[k]
+            cnt = np.bincount(s['row'], minlength=s['Av'].shape[0])
+            rowptr[o + 1:o + 1 + cnt.shape[0]] = cnt
+            if np.any(np.diff(s['row']) < 0):
+                raise ValueError('system rows must be sorted')
+            cols.append(s['col'] + o)
+            Ae.append(s['Ae'])
+            Av.append(s['Av'])
+            b.append(s['b'])
+        rowptr = np.cumsum(rowptr)
+        t32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+        self.rowptr, self.col = t32(rowptr), t32(np.concatenate(cols))
+        self.Ae, self.Av, self.b = f64(np.concatenate(Ae)), f64(np.concatenate(Av)), \
+            f64(np.concatenate(b))
+        self.diag = self.Av if jacobi else None
+        self.x = torch.zeros(N, dtype=torch.float64, device=dev)
+        self.r, self.z, self.p, self.Ap = (torch.empty_like(self.x) for _ in range(4))
+        self.state = torch.zeros(self.K * _STATE_DTYPE.itemsize, dtype=torch.uint8,
+                                 device=dev)
+        self.active = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.dev = dev
+        self.N = N
+        self.R = None
+        self.tiles = {}
+        self.graphs = {}
+
+    # -- tiling -------------------------------------------------------------
+    def _tiling(self, R):
+        if R not in self.tiles:
+            rows = 256 * R
+            inst, r0, r1, first, nt = [], [], [], [], []
+            for k in range(self.K):
+                o, n = int(self.offsets[k]), int(self.sizes[k])
+                first.append(len(inst))
+                cnt = (n + rows - 1) // rows
+                nt.append(cnt)
+                for t in range(cnt):
+                    inst.append(k)
+                    r0.append(o + t * rows)
+                    r1.append(min(o + n, o + (t + 1) * rows))
+            t32 = lambda a: torch.tensor(a, dtype=torch.int32, device=self.dev)
+            ntiles = len(inst)
+            self.tiles[R] = dict(n=ntiles, inst=t32(inst), row0=t32(r0), row1=t32(r1),
+                                 first=t32(first), ntiles=t32(nt),
+                                 part=torch.empty(2 * ntiles, dtype=torch.float64,
+                                                  device=self.dev))
+        return self.tiles[R]
+
+    def _tile_ptrs(self, R):
+        t = self._tiling(R)
+        return [R, t['n'], t['inst'].data_ptr(), t['row0'].data_ptr(),
+                t['row1'].data_ptr(), t['first'].data_ptr(), t['ntiles'].data_ptr()]
+
+    def _op(self):
+        return [self.rowptr.data_ptr(), self.col.data_ptr(), self.Ae.data_ptr(),
+                self.Av.data_ptr(), None if self.diag is None else self.diag.data_ptr()]
+
+    def _init(self, R, rtol, atol, maxiter):
+        st = np.zeros(self.K, _STATE_DTYPE)
+        st['rtol'] = rtol
+        st['atol'] = atol if atol else 0.0
+        st['maxiter'] = -1 if maxiter is None else maxiter
+        self.state.copy_(torch.from_numpy(st.view(np.uint8)).to(self.dev))
+        ops.fill_(self.active, self.K)
+        t = self._tiling(R)
+        nat.call('esr_bcg_init', *self._tile_ptrs(R), *self._op(), self.b.data_ptr(),
+                 self.x.data_ptr(), self.r.data_ptr(), self.z.data_ptr(),
+                 self.p.data_ptr(), self.state.data_ptr(), t['part'].data_ptr(),
+                 nat.stream_ptr())
+
+    def _steps(self, R, count, sp):
+        t = self._tiling(R)
+        nat.call('esr_bcg_steps', *self._tile_ptrs(R), *self._op(), self.x.data_ptr(),
+                 self.r.data_ptr(), self.z.data_ptr(), self.p.data_ptr(),
+                 self.Ap.data_ptr(), self.state.data_ptr(), t['part'].data_ptr(),
+                 self.active.data_ptr(), count, sp)
+
+    def autotune(self, iters=5):
+        """Pick the tile height with the fastest iterations on this batch."""
+        best = None
+        timings = {}
+        for R in _CANDIDATES:
+            ops.fill_(self.x, 0)
+            self._init(R, 0.0, None, None)
+            self._steps(R, 1, nat.stream_ptr())      # warm-up
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            self._steps(R, iters, nat.stream_ptr())
+            b.record()
+            torch.cuda.synchronize()
+            timings[R] = a.elapsed_time(b) / iters
+            if best is None or timings[R] < timings[best]:
+                best = R
+        self.R = best
+        self.autotune_ms = timings
+        ops.fill_(self.x, 0)
+        return best, timings
+
+    def solve(self, rtol=1e-8, atol=None, maxiter=None, chunk=32):
+        """Solve every instance from x = 0; returns a list of
+        {'residual', 'iters'} (cg.py:137 per instance)."""
+        R = self.R or 4
+        ops.fill_(self.x, 0)
+        self._init(R, rtol, atol, maxiter)
+        key = (R, chunk)
+        if key not in self.graphs:
+            self.graphs[key] = _Graph(lambda sp: self._steps(R, chunk, sp))
+        g = self.graphs[key]
+        while True:
+            g.replay()
+            if int(self.active.item()) == 0:
+                break
+        st = self.state.cpu().numpy().view(_STATE_DTYPE)
+        return [{'residual': float(st['rnorm'][k]), 'iters': int(st['iters'][k])}
+                for k in range(self.K)]
+
+    def solutions(self):
+        x = self.x.cpu().numpy()
+        return [x[self.offsets[k]:self.offsets[k + 1]] for k in range(self.K)]

@@ -40,7 +40,7 @@ nat.register_signatures({
     'esr_gmres_workspace': (_c_i32, [_c_i64, _c_i64, _P_i64, _P_i64]),
     'esr_gmres_begin': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_gmres_residual': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 9 + [_c_vp]),
-    'esr_gmres_cycle': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 12 + [_c_vp]),
+    'esr_gmres_cycle': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 11 + [_c_vp]),
     'esr_gmres_update_x': (_c_i32, [_c_i32, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_gmres_read': (_c_i32, [_c_vp, _c_i64, _P_dbl, _P_i64, _P_dbl, _c_vp]),
 })

@@ -1,0 +1,138 @@
+"""Mixed-precision conjugate gradients by iterative refinement.
+
+No reference counterpart (the reference's CG, numeric/solver/cg.py, runs in
+the tensors' dtype). The outer loop is float64:
+
+    r = b - A x                      (float64 operator, ||r|| in float64)
+    stop when ||r|| <= max(rtol ||b||, atol)
+    solve A d = r to `inner_rtol` with Jacobi-PCG in float32
+        (SpMV, axpy, vectors and operator in float32; dot products in float64)
+    x += d                            (float64)
+
+so convergence is always judged on the true float64 residual. The inner
+solve is the fused CG iteration (solver/fused.py) on float32 copies of the
+operator, which halves the bytes an iteration moves.
+"""
+import ctypes
+from typing import Any, Dict, Optional
+
+import torch
+from torch import nn
+
+from ... import _native as nat
+from ...core import module as esr
+from ...core.utils import logger
+from .fused import LinsysCSR, _Graph, _precond_diag
+
+_c_i32, _c_i64, _c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+nat.register_signatures({
+    'esr_ir_workspace': (_c_i32, [_c_i64, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    'esr_ir_residual': (_c_i32, [_c_i64] + [_c_vp] * 9 + [_c_i32, _c_vp]),
+    'esr_ir_update': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
+    'esr_ir_read': (_c_i32, [_c_vp, ctypes.POINTER(ctypes.c_double), _c_vp]),
+})
+
+
+class MixedPrecisionCG(esr.Module):
+    """CG by float64 iterative refinement around a float32 inner CG.
+
+    Same constructor as CG (A: a Linsys in float64, b, x: float64 partition
+    tensors, M: None or preconditioner.Jacobi)."""
+
+    def __init__(self, A: nn.Module, b: esr.Tensor, x: esr.Tensor,
+                 M: Optional[nn.Module] = None) -> None:
+        super().__init__()
+        self.A = A
+        self.b = b
+        self.x = x
+        self.M = M
+        self.easier_state = None
+
+    def easier_after_compile(self, prog):
+        why = LinsysCSR.applicable(self.A, self.x, prog)
+        ok, diag = _precond_diag(self.M, self.x, prog)
+        if why or not ok or self.x.dtype != torch.float64:
+            raise RuntimeError('MixedPrecisionCG needs a float64 Linsys and an '
+                               f'identity/Jacobi preconditioner ({why})')
+        op = LinsysCSR(self.A)
+        dev = self.x.device
+        n = op.n
+        f32 = lambda t: None if t is None else t.to(torch.float32)
+        self.easier_state = s = dict(op=op, n=n, diag64=diag)
+        s['Ae32'], s['Av32'], s['diag32'] = f32(op.Ae), f32(op.Av), f32(diag)
+        for k in ('r32', 'd32', 'ri', 'z', 'p', 'Ap'):
+            s[k] = torch.zeros(n, dtype=torch.float32, device=dev)
+        sb, wb = ctypes.c_int64(), ctypes.c_int64()
+        nat.call('esr_pcg_workspace', n, ctypes.byref(sb), ctypes.byref(wb))
+        s['pcg_state'] = torch.empty(sb.value, dtype=torch.uint8, device=dev)
+        s['pcg_ws'] = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+        nat.call('esr_ir_workspace', n, ctypes.byref(sb), ctypes.byref(wb))
+        s['ir_state'] = torch.empty(sb.value, dtype=torch.uint8, device=dev)
+        s['ir_ws'] = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+        s['graph'] = None
+
+    def _inner_arrays(self):
+        s = self.easier_state
+        op = s['op']
+        return [op.rowptr.data_ptr(), op.col.data_ptr(), s['Ae32'].data_ptr(),
+                s['Av32'].data_ptr(),
+                None if s['diag32'] is None else s['diag32'].data_ptr()]
+
+    def _inner_steps(self, count, sp):
+        s = self.easier_state
+        nat.call('esr_pcg_steps', nat.F32, s['n'], *self._inner_arrays(),
+                 s['d32'].data_ptr(), s['ri'].data_ptr(), s['z'].data_ptr(),
+                 s['p'].data_ptr(), s['Ap'].data_ptr(), s['pcg_state'].data_ptr(),
+                 s['pcg_ws'].data_ptr(), count, sp)
+
+    def _residual(self, first):
+        s = self.easier_state
+        op = s['op']
+        nat.call('esr_ir_residual', s['n'], op.rowptr.data_ptr(), op.col.data_ptr(),
+                 op.Ae.data_ptr(), op.Av.data_ptr(), self.b.data.data_ptr(),
+                 self.x.data.data_ptr(), s['r32'].data_ptr(), s['ir_state'].data_ptr(),
+                 s['ir_ws'].data_ptr(), 1 if first else 0, nat.stream_ptr())
+        sc = (ctypes.c_double * 2)()
+        nat.call('esr_ir_read', s['ir_state'].data_ptr(), sc, nat.stream_ptr())
+        return sc[0], sc[1]
+
+    def solve(self, rtol: float = 1e-10, atol: Optional[float] = None,
+              maxiter: Optional[int] = None, inner_rtol: float = 1e-4,
+              max_outer: int = 50, chunk: int = 32,
+              debug_iter: Optional[int] = None) -> Dict[str, Any]:
+        """Returns {'residual': true float64 ||b - A x||, 'iters': total inner
+        iterations, 'outer': refinement steps}. maxiter caps each inner solve."""
+        if self.easier_state is None:
+            raise RuntimeError('run esr.compile() first')
+        s = self.easier_state
+        if s['graph'] is None or s.get('chunk') != chunk:
+            s['graph'] = _Graph(lambda sp: self._inner_steps(chunk, sp))
+            s['chunk'] = chunk
+        rnorm, bnorm = self._residual(True)
+        tol = max(rtol * bnorm, atol) if atol else rtol * bnorm
+        total, outer = 0, 0
+        sc = (ctypes.c_double * 6)()
+        it = (ctypes.c_int64 * 3)()
+        while not (rnorm == rnorm and rnorm <= tol) and outer < max_outer:
+            outer += 1
+            # inner float32 Jacobi-PCG on A d = r from d = 0
+            nat.call('esr_pcg_init', nat.F32, s['n'], *self._inner_arrays(),
+                     s['r32'].data_ptr(), s['d32'].data_ptr(), s['ri'].data_ptr(),
+                     s['z'].data_ptr(), s['p'].data_ptr(), s['pcg_state'].data_ptr(),
+                     s['pcg_ws'].data_ptr(), float(inner_rtol), 0.0,
+                     -1 if maxiter is None else int(maxiter), nat.stream_ptr())
+            while True:
+                s['graph'].replay()
+                nat.call('esr_pcg_read', s['pcg_state'].data_ptr(), sc, it,
+                         nat.stream_ptr())
+                if it[1]:
+                    break
+            total += it[0] + 1
+            nat.call('esr_ir_update', s['n'], self.x.data.data_ptr(),
+                     s['d32'].data_ptr(), nat.stream_ptr())
+            rnorm, _ = self._residual(False)
+            if debug_iter is not None:
+                logger.info(f'MixedPrecisionCG outer {outer}: inner {it[0] + 1}, '
+                            f'true residual {rnorm}')
+        return {'residual': rnorm, 'iters': total, 'outer': outer,
+                'bnorm': bnorm}

@@ -285,6 +285,44 @@ int esr_gmres_update_x(int dtype, int64_t n, int64_t jcount, const void *V,
 int esr_gmres_read(const void *state, int64_t restart, double *scalars,
                    int64_t *ints, double *H, void *stream);
 
+/* ---- mixed-precision iterative refinement (new; no reference counterpart:
+ * the reference runs every solver in the tensors' one dtype) ----------------
+ * Outer loop in float64 around the float32 fused CG above:
+ * esr_ir_residual: r = b - A x in float64 on the float64 operator, stores r
+ * rounded to float32 and ||r|| (and ||b|| when first != 0) in the state;
+ * esr_ir_update: x += (double) d, then d = 0; esr_ir_read: {rnorm, bnorm}. */
+int esr_ir_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes);
+int esr_ir_residual(int64_t n, const int32_t *rowptr, const int32_t *col,
+                    const double *Ae, const double *Av, const double *b,
+                    const double *x, float *r32, void *state, void *ws,
+                    int first, void *stream);
+int esr_ir_update(int64_t n, double *x, float *d, void *stream);
+int esr_ir_read(const void *state, double *scalars, void *stream);
+
+/* ---- batched Jacobi-PCG (new; the reference solves one system per CG) -----
+ * cg.py:12-137 on K independent float64 Linsys instances concatenated into
+ * one CSR (global rows and columns). Tiles of 256*R rows (R in 1, 2, 4, 8)
+ * never straddle an instance: tinst/trow0/trow1 per tile, ifirst/intiles per
+ * instance. state: K records of esr_bcg_state_bytes() bytes whose rtol, atol
+ * and maxiter the caller sets before esr_bcg_init; `active` (int32) counts
+ * instances not yet converged (the caller sets it to K) and each converged
+ * instance's workgroups skip their tiles. part: 2 * ntiles doubles.       */
+int esr_bcg_state_bytes(int64_t *bytes);
+int esr_bcg_init(int R, int64_t ntiles, const int32_t *tinst,
+                 const int32_t *trow0, const int32_t *trow1,
+                 const int32_t *ifirst, const int32_t *intiles,
+                 const int32_t *rowptr, const int32_t *col, const double *Ae,
+                 const double *Av, const double *diag, const double *b,
+                 double *x, double *r, double *z, double *p, void *state,
+                 double *part, void *stream);
+int esr_bcg_steps(int R, int64_t ntiles, const int32_t *tinst,
+                  const int32_t *trow0, const int32_t *trow1,
+                  const int32_t *ifirst, const int32_t *intiles,
+                  const int32_t *rowptr, const int32_t *col, const double *Ae,
+                  const double *Av, const double *diag, double *x, double *r,
+                  double *z, double *p, double *Ap, void *state, double *part,
+                  int32_t *active, int64_t count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,3 +176,27 @@ def test_fused_cg_graph_replays_are_deterministic(gpu):
     sol.easier_fused = fused
     assert abs(info3['iters'] - info1['iters']) <= 1
     torch.testing.assert_close(eqn.x.collect(), x1, rtol=1e-8, atol=1e-10)
+
+
+def test_mixed_precision_cg_matches_fp64_oracle(gpu):
+    from easier_amd.numeric.solver import MixedPrecisionCG
+    from easier_amd.workloads import poisson as wp
+    mesh, sys_, _ = wp.varcoef_system(24, seed=5)
+    eqn = wp.PoissonP1(sys_)
+    sol = MixedPrecisionCG(eqn.A, eqn.b, eqn.x, eqn.M)
+    esr.compile([eqn, sol], 'hip')
+    info = sol.solve(rtol=1e-10, inner_rtol=1e-4, maxiter=10000)
+    assert info['outer'] >= 2
+    assert info['residual'] <= 1e-10 * info['bnorm']
+    A = osol.Linsys(torch.from_numpy(sys_['Av']), torch.from_numpy(sys_['Ae']),
+                    torch.from_numpy(sys_['col']), torch.from_numpy(sys_['row']),
+                    sys_['Av'].shape[0])
+    b = torch.from_numpy(sys_['b'])
+    x_o = torch.zeros_like(b)
+    osol.cg(A, b, x_o, M=lambda r: r / A.Av, rtol=1e-12, maxiter=20000)
+    x = eqn.x.collect()
+    err = float((x - x_o).norm() / x_o.norm())
+    assert err < 1e-7, err
+    # the true residual the solver reports is the float64 one
+    r = b - A(x)
+    torch.testing.assert_close(torch.tensor(info['residual']), r.norm(), rtol=1e-6, atol=0)
