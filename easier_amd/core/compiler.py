@@ -454,7 +454,9 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
         e = sel_out[s]
         if e in order or e in reducer_inputs or e in prog.bound_reducer:
             continue
-        n_in = space_len[sel_in[s]]
+        # a Selector never applied in a traced forward has no known input
+        # length: bound its index range by its own maximum
+        n_in = space_len.get(sel_in[s], int(sel_idx[s].max()) + 1)
         inv = inv_of(sel_in[s])
         key = sel_idx[s] if inv is None else inv[sel_idx[s]]
         _, perm = csr_from_idx(key, n_in)

@@ -333,6 +333,27 @@ __device__ __forceinline__ i64 esr_remap(i64 b, i64 nb) {
   const i64 x = b % ESR_XCD, pos = b / ESR_XCD;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
 }
+// correctly rounded sqrt: device sqrt, then the Tuckerman rounding test
+__device__ __forceinline__ double esr_sqrt(double x) {
+  const double y = sqrt(x);
+  if (!(x > 0x1p-900) || x > 0x1p+1000) return y;
+  const long long yb = __builtin_bit_cast(long long, y);
+  const double yu = __builtin_bit_cast(double, yb + 1);
+  const double yd = __builtin_bit_cast(double, yb - 1);
+  if (__builtin_fma(-y, yu, x) > 0.0) return yu;
+  if (__builtin_fma(-y, yd, x) <= 0.0) return yd;
+  return y;
+}
+__device__ __forceinline__ float esr_sqrt(float x) {
+  const float y = sqrtf(x);
+  if (!(x > 0x1p-100f) || x > 0x1p+100f) return y;
+  const int yb = __builtin_bit_cast(int, y);
+  const float yu = __builtin_bit_cast(float, yb + 1);
+  const float yd = __builtin_bit_cast(float, yb - 1);
+  if (__builtin_fmaf(-y, yu, x) > 0.0f) return yu;
+  if (__builtin_fmaf(-y, yd, x) <= 0.0f) return yd;
+  return y;
+}
 '''
 
 
@@ -459,7 +480,9 @@ class _Lowering:
             if op == 'neg': return f'(-{a[0]})'
             if op == 'bcast': return a[0]
             if op == 'abs': return f'fabs({a[0]})'
-            if op in ('exp', 'sqrt', 'sin', 'cos', 'tanh', 'log'):
+            if op == 'sqrt':
+                return f'esr_sqrt({a[0]})'
+            if op in ('exp', 'sin', 'cos', 'tanh', 'log'):
                 return f'{op}({a[0]})'
             if op == 'sign':
                 return (f'(({a[0]}) > ({ct})0 ? ({ct})1 : (({a[0]}) < ({ct})0 ? '
@@ -468,7 +491,7 @@ class _Lowering:
                 e = v.args[1].const
                 if e == 2: return f'({a[0]} * {a[0]})'
                 if e == 3: return f'({a[0]} * {a[0]} * {a[0]})'
-                if e == 0.5: return f'sqrt({a[0]})'
+                if e == 0.5: return f'esr_sqrt({a[0]})'
                 if e == -1: return f'(({ct})1 / {a[0]})'
                 return f'pow({a[0]}, {a[1]})'
             if op in _PRED:

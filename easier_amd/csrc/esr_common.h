@@ -44,6 +44,35 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return start + pos;
 }
 
+// IEEE correctly rounded square root. The device sqrt can be off by one ulp;
+// given y within one ulp of sqrt(x), the Tuckerman rounding test (Markstein,
+// "IA-64 and Elementary Functions", 2000) picks the correctly rounded
+// neighbour with two exact-sign FMA residuals: round up when x > y * y_up,
+// down when x <= y * y_down (the half-ulp-squared terms of the exact test
+// are below the granularity of x - y * y_neighbour).
+// (x restricted to a range where y is a positive normal number, so the
+// neighbours are the bit patterns one above and one below.)
+__device__ __forceinline__ double cr_sqrt(double x) {
+  const double y = sqrt(x);
+  if (!(x > 0x1p-900) || x > 0x1p+1000) return y;
+  const long long yb = __builtin_bit_cast(long long, y);
+  const double yu = __builtin_bit_cast(double, yb + 1);
+  const double yd = __builtin_bit_cast(double, yb - 1);
+  if (__builtin_fma(-y, yu, x) > 0.0) return yu;
+  if (__builtin_fma(-y, yd, x) <= 0.0) return yd;
+  return y;
+}
+__device__ __forceinline__ float cr_sqrt(float x) {
+  const float y = sqrtf(x);
+  if (!(x > 0x1p-100f) || x > 0x1p+100f) return y;
+  const int yb = __builtin_bit_cast(int, y);
+  const float yu = __builtin_bit_cast(float, yb + 1);
+  const float yd = __builtin_bit_cast(float, yb - 1);
+  if (__builtin_fmaf(-y, yu, x) > 0.0f) return yu;
+  if (__builtin_fmaf(-y, yd, x) <= 0.0f) return yd;
+  return y;
+}
+
 } // namespace esr
 
 #define ESR_API extern "C" __attribute__((visibility("default")))

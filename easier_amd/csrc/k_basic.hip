@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_stage2(const T *__restrict__ par
   acc = block_reduce(acc, cmb, cmb.identity());
   if (threadIdx.x == 0) {
     if (op == ESR_AGG_NORM && !isinf(p)) {
-      if (p == 2.0) acc = (T)sqrt((double)acc);
+      if (p == 2.0) acc = (T)cr_sqrt((double)acc);
       else if (p != 1.0 && p != 0.0) acc = (T)pow((double)acc, 1.0 / p);
     }
     out[c] = acc;
@@ -347,8 +347,8 @@ template <typename T> __device__ __forceinline__ T ew_unary(int op, T a) {
     case ESR_EW_ABS: return a < T(0) ? -a : (a == T(0) ? T(0) : a);
     case ESR_EW_EXP: return exp(a);
     case ESR_EW_LOG: return log(a);
-    case ESR_EW_SQRT: return sqrt(a);
-    case ESR_EW_RSQRT: return T(1) / sqrt(a);
+    case ESR_EW_SQRT: return cr_sqrt(a);
+    case ESR_EW_RSQRT: return T(1) / cr_sqrt(a);
     case ESR_EW_SIN: return sin(a);
     case ESR_EW_COS: return cos(a);
     case ESR_EW_TANH: return tanh(a);
@@ -377,8 +377,8 @@ template <typename T> __device__ __forceinline__ T ew_pow(T a, T b) {
     // (an exponent of 2 or 3 is evaluated by multiplication).
     if (b == T(2)) return a * a;
     if (b == T(3)) return a * a * a;
-    if (b == T(0.5)) return sqrt(a);
-    if (b == T(-0.5)) return T(1) / sqrt(a);
+    if (b == T(0.5)) return cr_sqrt(a);
+    if (b == T(-0.5)) return T(1) / cr_sqrt(a);
     if (b == T(-1)) return T(1) / a;
     if (b == T(-2)) return T(1) / (a * a);
     return pow(a, b);

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kB) void k_b_init(
   if (inst_sum_last<2>(acc, part, ntiles, t, tl.first[k], tl.ntiles[k], &s->counter[0], tot) &&
       threadIdx.x == 0) {
     s->rz_cur = s->rz_next = tot[0];
-    s->bnorm = sqrt(tot[1]);
+    s->bnorm = cr_sqrt(tot[1]);
     double tol = s->rtol * s->bnorm;
     if (s->atol > 0.0 && s->atol > tol) tol = s->atol;
     s->tol = tol;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kB) void k_b_xr(Tiles tl, int64_t ntiles,
       threadIdx.x == 0) {
     s->rr = tot[0];
     s->rz_new = tot[1];
-    s->rnorm = sqrt(tot[0]);
+    s->rnorm = cr_sqrt(tot[0]);
   }
 }
 

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kB) void k_pcg_init(
   if (grid_sum_last<2>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
     st->rz_cur = tot[0];
     st->rz_next = tot[0];
-    st->bnorm = sqrt(tot[1]);
+    st->bnorm = cr_sqrt(tot[1]);
     double tol = st->rtol * st->bnorm;
     if (st->atol > 0.0 && st->atol > tol) tol = st->atol;
     st->tol = tol;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kB) void k_pcg_xr(int64_t n, const T *__restrict__ 
   if (grid_sum_last<2>(acc, part, &st->counter[2], gridDim.x, b, tot) && threadIdx.x == 0) {
     st->rr = tot[0];
     st->rz_new = tot[1];
-    st->rnorm = sqrt(tot[0]);
+    st->rnorm = cr_sqrt(tot[0]);
   }
 }
 
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kB) void k_gm_bnorm(int64_t n, const T *__restrict_
   }
   double tot[1];
   if (grid_sum_last<1>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0)
-    st->bnorm = sqrt(tot[0]);
+    st->bnorm = cr_sqrt(tot[0]);
 }
 
 // UpdateRnorm (gmres.py:22-25): r = b - A x; rnorm = ||r||. Clears the
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kB) void k_gm_residual(
   }
   double tot[1];
   if (grid_sum_last<1>(acc, part, &st->counter[1], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->rnorm = sqrt(tot[0]);
+    st->rnorm = cr_sqrt(tot[0]);
     st->broken = 0;
     st->jend = -1;
   }
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kB) void k_gm_norm(int64_t n, int j, const T *__res
   }
   double tot[1];
   if (grid_sum_last<1>(acc, part, &st->counter[3], gridDim.x, b, tot) && threadIdx.x == 0) {
-    const double hn = (double)(T)sqrt(tot[0]);
+    const double hn = (double)(T)cr_sqrt(tot[0]);
     st->H[(j + 1) * kGmresMaxRestart + j] = hn;
     st->jend = j;
     if (hn < 1e-15) st->broken = 1;
@@ -376,8 +376,8 @@ __global__ __launch_bounds__(kB) void k_ir_residual(
   }
   double tot[2];
   if (grid_sum_last<2>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->rnorm = sqrt(tot[0]);
-    if (first) st->bnorm = sqrt(tot[1]);
+    st->rnorm = cr_sqrt(tot[0]);
+    if (first) st->bnorm = cr_sqrt(tot[1]);
   }
 }
 

@@ -247,7 +247,7 @@ class FusedGMRES:
         it = (ctypes.c_int64 * 2)()
         H = np.zeros((m + 1) * m) if with_H else None
         nat.call('esr_gmres_read', self.state.data_ptr(), m, sc, it,
-                 H.ctypes.data if with_H else None, nat.stream_ptr())
+                 H.ctypes.data_as(_P_dbl) if with_H else None, nat.stream_ptr())
         return sc[0], sc[1], it[0], it[1], (H.reshape(m + 1, m) if with_H else None)
 
     def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None):

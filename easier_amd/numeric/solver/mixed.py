@@ -33,6 +33,24 @@ nat.register_signatures({
 })
 
 
+class TrueResidual(esr.Module):
+    """r = b - A x in the tensors' dtype (the role of gmres.py's UpdateRnorm);
+    traced by esr.compile, it also ties A's index spaces to x."""
+
+    def __init__(self, A, b, x, r, rnorm):
+        super().__init__()
+        self.A = A
+        self.b = b
+        self.x = x
+        self.r = r
+        self.rnorm = rnorm
+
+    def forward(self):
+        r = self.b - self.A(self.x)
+        self.r[:] = r
+        self.rnorm[:] = esr.norm(r).norm()
+
+
 class MixedPrecisionCG(esr.Module):
     """CG by float64 iterative refinement around a float32 inner CG.
 
@@ -46,6 +64,10 @@ class MixedPrecisionCG(esr.Module):
         self.b = b
         self.x = x
         self.M = M
+        self.r = esr.Tensor(esr.zeros_like(x), mode='partition')
+        self.rnorm = esr.Tensor(torch.tensor([0.0], dtype=x.dtype),
+                                mode='replicate')
+        self.residual = TrueResidual(A, b, x, self.r, self.rnorm)
         self.easier_state = None
 
     def easier_after_compile(self, prog):
