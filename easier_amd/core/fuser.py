@@ -580,13 +580,33 @@ class _Lowering:
             for u in members:
                 accs.append(f'acc{u.id}')
             inner = [emit_edge(u.args[0]) for u in members]
-            for u, a in zip(members, accs):
-                body.append(f'  {a} = {a} + {inner[members.index(u)]};')
             rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
-            rows.append(f'  for (int j = {rp}[r]; j < {rp}[r + 1]; ++j) {{')
-            rows.append(f'    const int e = {perm + "[j]" if perm else "j"};')
-            rows.extend('  ' + b for b in body)
-            rows.append('  }')
+            hp = red.easier_host_rowptr
+            counts = np.diff(hp)
+            kmax = int(counts.max()) if counts.size else 0
+            dense = 0 < kmax <= 8 and counts.mean() >= 0.5 * kmax
+            if dense:
+                # every row has at most kmax entries: evaluate kmax slots with
+                # clamped (always valid) edge positions so all their loads
+                # issue together, and fold slot k into the sums only when it
+                # exists -- the same left-to-right order as the loop
+                for u, a in zip(members, accs):
+                    body.append(f'  {a} = (j < je) ? {a} + {inner[members.index(u)]} : {a};')
+                rows.append(f'  {{ const int jb = {rp}[r], je = {rp}[r + 1];')
+                rows.append(f'  #pragma unroll')
+                rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
+                rows.append('    const int j = jb + k;')
+                rows.append('    const int jc = j < je ? j : (je > 0 ? je - 1 : 0);')
+                rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
+                rows.extend('  ' + b for b in body)
+                rows.append('  } }')
+            else:
+                for u, a in zip(members, accs):
+                    body.append(f'  {a} = {a} + {inner[members.index(u)]};')
+                rows.append(f'  for (int j = {rp}[r]; j < {rp}[r + 1]; ++j) {{')
+                rows.append(f'    const int e = {perm + "[j]" if perm else "j"};')
+                rows.extend('  ' + b for b in body)
+                rows.append('  }')
             for u, a in zip(members, accs):
                 memo[u.id] = a
 

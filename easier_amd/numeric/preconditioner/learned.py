@@ -1,0 +1,196 @@
+"""Learned preconditioner z = M(r): an edge message-passing network on the
+mesh graph, executed by hand-written kernels (csrc/k_gnn.hip).
+
+No reference counterpart (easier/numeric/preconditioner/__init__.py:4 is an
+empty slot; CG/GMRES accept any callable M, cg.py:86). The network is
+nonlinear in r, so it is used inside flexible CG (solver/fcg.py).
+
+Parameters live in one flat float32 device tensor (layout below, shared with
+the kernels); the matrices the matrix cores consume are packed to bfloat16
+after every update. GnnApply wraps forward + backward as a
+torch.autograd.Function (float32 gradients w.r.t. the flat parameters).
+
+Flat layout (H = 32): W_in (3, H), b_in (H); per layer Wp (H, H), Wq (H, H),
+We (3, H), bm (H), Wu (2H, H), bu (H); w_out (H), b_out (1). Matrices are
+row-major with the input index first (y = x W).
+"""
+import ctypes
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ...core import ops
+
+H = 32
+_c_i32, _c_i64, _c_vp, _c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
+nat.register_signatures({
+    'esr_gnn_sizes': (_c_i32, [_c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    'esr_gnn_pack': (_c_i32, [_c_i32, _c_vp, _c_vp, _c_vp]),
+    'esr_gnn_forward': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i32, _c_vp, _c_vp]),
+    'esr_gnn_backward': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 16 + [_c_i64, _c_vp]),
+    'esr_gnn_adam': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_dbl,
+                              _c_dbl, _c_i64, _c_dbl, _c_vp]),
+})
+
+
+def param_shapes(L):
+    shapes = [('W_in', (3, H)), ('b_in', (H,))]
+    for l in range(L):
+        shapes += [(f'Wp{l}', (H, H)), (f'Wq{l}', (H, H)), (f'We{l}', (3, H)),
+                   (f'bm{l}', (H,)), (f'Wu{l}', (2 * H, H)), (f'bu{l}', (H,))]
+    return shapes + [('w_out', (H,)), ('b_out', ())]
+
+
+def flatten(params: Dict[str, torch.Tensor], L) -> torch.Tensor:
+    return torch.cat([params[k].reshape(-1).float() for k, _ in param_shapes(L)])
+
+
+def unflatten(flat: torch.Tensor, L) -> Dict[str, torch.Tensor]:
+    out, o = {}, 0
+    for k, shp in param_shapes(L):
+        sz = int(np.prod(shp)) if shp else 1
+        out[k] = flat[o:o + sz].reshape(shp)
+        o += sz
+    return out
+
+
+class MeshGraph:
+    """Static inputs of the network for one operator: the CSR (row-sorted
+    directed vertex edges), the reverse edge of every edge, vertex features
+    (log(d/mean d), log kappa) and edge features (a_ij/sqrt(d_i d_j), edge
+    vector in grid units), and the diagonal d."""
+
+    def __init__(self, row, col, Av, Ae, points, logk_vertex, n_grid, device=None):
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        row = np.asarray(row, np.int64)
+        col = np.asarray(col, np.int64)
+        n = Av.shape[0]
+        if np.any(np.diff(row) < 0):
+            raise ValueError('edges must be sorted by row')
+        rowptr = np.concatenate([[0], np.cumsum(np.bincount(row, minlength=n))])
+        key = row * n + col
+        rkey = col * n + row
+        order = np.argsort(key)
+        pos = np.searchsorted(key[order], rkey)
+        if np.any(pos >= key.shape[0]) or np.any(key[order][np.minimum(pos, key.shape[0] - 1)] != rkey):
+            raise ValueError('the edge set is not symmetric')
+        rev = order[pos]
+        Av = np.asarray(Av, np.float64)
+        Ae = np.asarray(Ae, np.float64)
+        pts = np.asarray(points, np.float64)
+        vfeat = np.stack([np.log(Av / Av.mean()),
+                          np.asarray(logk_vertex, np.float64).reshape(-1)], 1).astype(np.float32)
+        efeat = np.stack([Ae / np.sqrt(Av[row] * Av[col]),
+                          (pts[col, 0] - pts[row, 0]) * n_grid,
+                          (pts[col, 1] - pts[row, 1]) * n_grid], 1).astype(np.float32)
+        i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        self.n, self.nnz = n, row.shape[0]
+        self.rowptr, self.col, self.rev = i32(rowptr), i32(col), i32(rev)
+        self.vfeat = torch.from_numpy(np.ascontiguousarray(vfeat)).to(dev)
+        self.efeat = torch.from_numpy(np.ascontiguousarray(efeat)).to(dev)
+        self.d = torch.from_numpy(Av).to(dev)
+        self.device = dev
+
+
+class LearnedPreconditioner:
+    def __init__(self, L=3, params: Optional[Dict[str, torch.Tensor]] = None,
+                 device=None):
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        npar, npk = ctypes.c_int64(), ctypes.c_int64()
+        nat.call('esr_gnn_sizes', L, ctypes.byref(npar), ctypes.byref(npk))
+        self.L, self.device = L, dev
+        if params is None:
+            from_oracle = None
+            params = {k: torch.zeros(s) for k, s in param_shapes(L)}
+        flat = flatten(params, L)
+        assert flat.numel() == npar.value, (flat.numel(), npar.value)
+        self.params = flat.to(dev).contiguous()
+        self.packed = torch.empty(npk.value, dtype=torch.bfloat16, device=dev)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.step_count = 0
+        self.graph: Optional[MeshGraph] = None
+        self.pack()
+
+    def pack(self):
+        nat.call('esr_gnn_pack', self.L, self.params.data_ptr(), self.packed.data_ptr(),
+                 nat.stream_ptr())
+
+    def named(self):
+        return unflatten(self.params.detach().cpu(), self.L)
+
+    # -- binding to a mesh --------------------------------------------------
+    def bind(self, graph: MeshGraph, train=False):
+        self.graph = graph
+        n, L = graph.n, self.L
+        nh = (L + 1) if train else 2
+        nq = L if train else 2
+        self.h = torch.empty((nh, n, H), dtype=torch.bfloat16, device=self.device)
+        self.Q = torch.empty((nq, n, H), dtype=torch.bfloat16, device=self.device)
+        self.a = torch.empty((L, n, H), dtype=torch.float32, device=self.device) if train else None
+        self.rnorm = torch.empty(1, dtype=torch.float64, device=self.device)
+        self.train = train
+        if train:
+            need = n * (8 * H + 3) + graph.nnz * H + 256 * 2 * H * H
+            self.work = torch.empty(need, dtype=torch.float32, device=self.device)
+        return self
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor):
+        """z = M(r) (float64 device vectors of the bound mesh)."""
+        g = self.graph
+        nrm = ops.aggregate('norm', r.reshape(-1, 1))
+        ops.copy_(self.rnorm, nrm.reshape(1))
+        nat.call('esr_gnn_forward', self.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
+                 g.efeat.data_ptr(), g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(),
+                 self.rnorm.data_ptr(), self.params.data_ptr(), self.packed.data_ptr(),
+                 self.h.data_ptr(), self.Q.data_ptr(),
+                 None if self.a is None else self.a.data_ptr(), 1 if self.train else 0,
+                 z.data_ptr(), nat.stream_ptr())
+        return z
+
+    def backward(self, r: torch.Tensor, dz: torch.Tensor) -> torch.Tensor:
+        """Gradient w.r.t. the flat parameters of sum(dz * M(r)); needs the
+        preceding apply(r, .) in training mode."""
+        if not self.train:
+            raise RuntimeError('bind(graph, train=True) before backward()')
+        g = self.graph
+        nat.call('esr_gnn_backward', self.L, g.n, g.nnz, g.rowptr.data_ptr(),
+                 g.col.data_ptr(), g.rev.data_ptr(), g.efeat.data_ptr(), g.vfeat.data_ptr(),
+                 r.data_ptr(), g.d.data_ptr(), self.rnorm.data_ptr(), self.params.data_ptr(),
+                 self.packed.data_ptr(), self.h.data_ptr(), self.Q.data_ptr(),
+                 self.a.data_ptr(), dz.data_ptr(), self.grad.data_ptr(),
+                 self.work.data_ptr(), self.work.numel(), nat.stream_ptr())
+        return self.grad
+
+    def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, clip=0.0):
+        self.step_count += 1
+        nat.call('esr_gnn_adam', self.params.numel(), self.params.data_ptr(),
+                 self.grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(), lr, b1, b2,
+                 eps, self.step_count, clip, nat.stream_ptr())
+        self.pack()
+
+    def __call__(self, r):
+        z = torch.empty_like(r)
+        return self.apply(r, z)
+
+
+class GnnApply(torch.autograd.Function):
+    """z = GnnApply.apply(params, r, pre): forward and backward run the
+    hand-written kernels; params must be pre.params (the flat tensor)."""
+
+    @staticmethod
+    def forward(ctx, params, r, pre: LearnedPreconditioner):
+        z = torch.empty_like(r)
+        pre.apply(r, z)
+        ctx.pre = pre
+        ctx.save_for_backward(r)
+        return z
+
+    @staticmethod
+    def backward(ctx, dz):
+        (r,) = ctx.saved_tensors
+        g = ctx.pre.backward(r, dz.contiguous().to(torch.float64))
+        return g.clone(), None, None

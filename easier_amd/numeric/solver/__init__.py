@@ -4,3 +4,5 @@ flexible CG."""
 from .cg import CG
 from .gmres import GMRES
 from .mixed import MixedPrecisionCG
+from .batched import BatchedCG
+from .fcg import FlexibleCG

@@ -323,6 +323,50 @@ int esr_bcg_steps(int R, int64_t ntiles, const int32_t *tinst,
                   double *z, double *p, double *Ap, void *state, double *part,
                   int32_t *active, int64_t count, void *stream);
 
+/* ---- learned preconditioner (new; the reference's preconditioner slot,
+ * easier/numeric/preconditioner/__init__.py:4, is empty) --------------------
+ * z = M(r), an edge message-passing network with L layers of width 32 (see
+ * easier_amd/numeric/preconditioner/learned.py for the flat parameter
+ * layout). esr_gnn_sizes: float32 parameter count and bfloat16 packed-matrix
+ * count; esr_gnn_pack: params -> packed (bf16 transposed Wp, Wq, Wu per
+ * layer, the matrix-core operands); esr_gnn_forward: h holds 2 (or L+1 when
+ * save) n x 32 bf16 buffers, Q 2 (or L), a L float32 n x 32 buffers when
+ * save (else NULL), rnorm = device ||r||; esr_gnn_backward: gradient of
+ * sum(dz * z) w.r.t. the parameters after a save=1 forward (rev = reverse
+ * edge of every edge, work = float32 scratch, size in the source);
+ * esr_gnn_adam: one Adam step (bias-corrected, optional gradient clip).     */
+int esr_gnn_sizes(int L, int64_t *n_params, int64_t *n_packed);
+int esr_gnn_pack(int L, const float *params, void *packed, void *stream);
+int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr,
+                    const int32_t *col, const float *efeat, const float *vfeat,
+                    const double *r, const double *d, const double *rnorm,
+                    const float *params, const void *packed, void *h, void *Q,
+                    float *a, int save, double *z, void *stream);
+int esr_gnn_backward(int L, int64_t n, int64_t nnz, const int32_t *rowptr,
+                     const int32_t *col, const int32_t *rev,
+                     const float *efeat, const float *vfeat, const double *r,
+                     const double *d, const double *rnorm,
+                     const float *params, const void *packed, const void *h,
+                     const void *Q, const float *a, const double *dz,
+                     float *grad, float *work, int64_t work_floats,
+                     void *stream);
+int esr_gnn_adam(int64_t np_, float *params, const float *grad, float *m,
+                 float *v, double lr, double b1, double b2, double eps,
+                 int64_t step, double clip, void *stream);
+
+/* ---- flexible CG (Polak-Ribiere) steps for a varying preconditioner ------
+ * op: 0 q = A p, p.q, alpha = rz / p.q; 1 r_old = r, r -= alpha q,
+ * x += alpha p, ||r||^2; 2 z.r, z.r_old, beta (a0 != 0: restart, beta = 0),
+ * rz = z.r; 3 p = z + beta p; 4 r = b - A x, ||r||^2, ||b||^2; 5 z = r / d.
+ * esr_fcg_read: {rz, pq, alpha, rr, zr, zro, beta, rr_true, bb}.          */
+int esr_fcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes);
+int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
+               const double *Ae, const double *Av, const double *d,
+               const double *b, double *x, double *r, double *ro, double *p,
+               double *q, double *z, int a0, void *state, void *ws,
+               void *stream);
+int esr_fcg_read(const void *state, double *scalars, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

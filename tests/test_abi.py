@@ -10,6 +10,8 @@ from easier_amd import _native as nat
 import easier_amd.numeric.solver.fused  # noqa: F401  (registers entry points)
 import easier_amd.numeric.solver.mixed  # noqa: F401
 import easier_amd.numeric.solver.batched  # noqa: F401
+import easier_amd.numeric.solver.fcg  # noqa: F401
+import easier_amd.numeric.preconditioner.learned  # noqa: F401
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 

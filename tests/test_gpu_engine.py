@@ -199,4 +199,5 @@ def test_mixed_precision_cg_matches_fp64_oracle(gpu):
     assert err < 1e-7, err
     # the true residual the solver reports is the float64 one
     r = b - A(x)
-    torch.testing.assert_close(torch.tensor(info['residual']), r.norm(), rtol=1e-6, atol=0)
+    torch.testing.assert_close(torch.tensor(info['residual'], dtype=torch.float64), r.norm(),
+                               rtol=1e-6, atol=0)
