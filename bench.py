@@ -151,6 +151,29 @@ def poisson_config1(skip_cpu):
     return out
 
 
+def copy_calibration(mib):
+    """dst = src for a float64 array of `mib` MiB, 5 launches: reads and
+    writes exactly mib MiB each; prints the event-timed bandwidth."""
+    import torch
+    from easier_amd.core import ops
+    n = mib * (1 << 20) // 8
+    src = torch.empty(n, dtype=torch.float64, device='cuda').uniform_()
+    dst = torch.empty_like(src)
+    ops.copy_(dst, src)
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    stream = torch.cuda.current_stream()
+    for i in range(5):
+        evs[i].record(stream)
+        ops.copy_(dst, src)
+    evs[5].record(stream)
+    torch.cuda.synchronize()
+    ts = [evs[i].elapsed_time(evs[i + 1]) / 1e3 for i in range(5)]
+    t = sorted(ts)[2]
+    print(json.dumps({'copy_bytes_each_way': n * 8, 'median_s': t,
+                      'read_plus_write_GBps': 2 * n * 8 / t / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -165,6 +188,9 @@ def main():
     ap.add_argument('--no-fuse', action='store_true')
     ap.add_argument('--profile', action='store_true',
                     help='only warm-up + timed steps (for rocprofv3 runs)')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only copy a float64 array of MIB MiB 5 times (the known-size '
+                         'kernel that scales the HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py: this build runs on one GPU; --gpus must be 1 '
@@ -174,6 +200,9 @@ def main():
     if not torch.cuda.is_available():
         print('bench.py: no GPU visible', file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     dtype = torch.float64 if args.dtype == 'f64' else torch.float32
     T = 8 if dtype == torch.float64 else 4
     mesh, geo, eqn, meta = build_swe(args.n, dtype, args.seed, fuse=not args.no_fuse)
@@ -199,6 +228,11 @@ def main():
     h = eqn.h.collect()
     finite = bool(torch.isfinite(h).all())
     stages, step_bytes = swe_bytes(nc, ne, nbc, T)
+    kerns = getattr(eqn.easier_engine, 'kernels', None)
+    if kerns:
+        # compulsory bytes of each generated kernel (its arrays, once each)
+        stages = [k.algorithmic_bytes() for k in kerns]
+        step_bytes = sum(stages)
     result = {
         'metric': METRIC, 'value': value, 'unit': 'cells/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
@@ -211,6 +245,7 @@ def main():
                         'reflective walls',
             'cells': nc, 'edges': ne, 'engine': type(eqn.easier_engine).__name__,
             'step_bytes_algorithmic': step_bytes,
+            'step_bytes_formula': swe_bytes(nc, ne, nbc, T)[1],
             'step_hbm_fraction': step_bytes / (wall / args.steps) / HBM_PEAK,
             'state_finite': finite,
         },

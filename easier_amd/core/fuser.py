@@ -47,6 +47,18 @@ class Unsupported(Exception):
     pass
 
 
+# Code-generation options of the row kernels (all keep the arithmetic and its
+# order unchanged):
+#   stage_edges  - a dense Reducer group (at most 8 edges per row) first copies
+#                  the block's contiguous edge range of every edge array it
+#                  reads into LDS with coalesced loads, then reads the rows'
+#                  edges from LDS;
+#   nontemporal  - edge-array loads and row stores carry the nontemporal hint
+#                  (streamed once, not kept in the caches).
+OPTIONS = {'stage_edges': False, 'nontemporal': False}
+_LDS_BUDGET = 48 * 1024
+
+
 _EW_BIN = {
     operator.add: 'add', operator.sub: 'sub', operator.mul: 'mul',
     operator.truediv: 'div', operator.pow: 'pow',
@@ -357,13 +369,30 @@ __device__ __forceinline__ float esr_sqrt(float x) {
 '''
 
 
+class _Restage(Exception):
+    """The staged edge arrays of a kernel exceed the LDS budget."""
+
+
 class Kernel:
     def __init__(self, space, stage, name):
         self.space, self.stage, self.name = space, stage, name
         self.params: List[tuple] = []     # (cname, ctype, tensor)
         self.pindex: Dict[int, str] = {}
+        self.access: Dict[str, set] = {}  # cname -> {'r', 'w'}
         self.n = 0
         self.src = ''
+        self.staged = False
+        self.lds_bytes = 0
+
+    def algorithmic_bytes(self) -> int:
+        """Compulsory HBM bytes of one launch: every array the kernel reads
+        counted once, every array it writes counted once (read-modify-write
+        arrays twice); gathered source rows are counted once, as a read of
+        the whole source array."""
+        tot = 0
+        for name, _, t in self.params:
+            tot += t.numel() * t.element_size() * len(self.access.get(name, ()))
+        return tot
 
 
 class FusedEngine:
@@ -400,6 +429,7 @@ class _Lowering:
         self.ctype = 'double' if p.dtype == torch.float64 else 'float'
         self.device = prog.device
         self.kernels: List[Kernel] = []
+        self.no_stage = False
 
     def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
         a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
@@ -430,7 +460,16 @@ class _Lowering:
             roots = groups[(stage, space)]
             kern = Kernel(space, stage, f'esr_fused_{k}')
             kern.n = self.prog.space_len[space]
-            self._emit_kernel(kern, roots)
+            try:
+                self._emit_kernel(kern, roots)
+            except _Restage:
+                # too many staged edge arrays for the LDS budget: this
+                # kernel reads its edges straight from global memory
+                self.no_stage = True
+                kern = Kernel(space, stage, f'esr_fused_{k}')
+                kern.n = self.prog.space_len[space]
+                self._emit_kernel(kern, roots)
+                self.no_stage = False
             self.kernels.append(kern)
             srcs.append(kern.src)
         return _CODE_HEADER + '\n'.join(srcs)
@@ -444,10 +483,11 @@ class _Lowering:
                 k = int(name[3:])
                 n_, _, t_ = kern.params[k]
                 kern.params[k] = (n_, ctype + '*', t_)
-            return name
-        name = f'a.p{len(kern.params)}'
-        kern.params.append((name, ('const ' if const else '') + ctype + '*', tensor))
-        kern.pindex[key] = name
+        else:
+            name = f'a.p{len(kern.params)}'
+            kern.params.append((name, ('const ' if const else '') + ctype + '*', tensor))
+            kern.pindex[key] = name
+        kern.access.setdefault(name, set()).add('r' if const else 'w')
         return name
 
     def _emit_kernel(self, kern: Kernel, roots: List[V]):
@@ -549,6 +589,24 @@ class _Lowering:
                 self._param(kern, red.easier_perm, 'int')
             body: List[str] = []
             emem: Dict[int, str] = {}
+            hp = red.easier_host_rowptr
+            counts = np.diff(hp)
+            kmax = int(counts.max()) if counts.size else 0
+            dense = 0 < kmax <= 8 and counts.mean() >= 0.5 * kmax
+            gid = len(done_groups)
+            # edge arrays staged through LDS: param name -> (lds name, ctype)
+            staged: Dict[str, tuple] = {}
+            stage = dense and perm is None and OPTIONS['stage_edges'] and not self.no_stage
+            nt = OPTIONS['nontemporal']
+
+            def edge_load(name, ctype):
+                if stage:
+                    if name not in staged:
+                        staged[name] = (f's{gid}_{len(staged)}', ctype)
+                    return f'{staged[name][0]}[jl]'
+                if nt:
+                    return f'__builtin_nontemporal_load(&{name}[e])'
+                return f'{name}[e]'
 
             def emit_edge(x: V) -> str:
                 if x.id in emem:
@@ -556,7 +614,7 @@ class _Lowering:
                 if x.space != v.args[0].space:
                     s = emit_row(x)           # row value or constant, hoisted
                 elif x.kind == 'leaf':
-                    s = var(body, f'{buf_of(x)}[e]')
+                    s = var(body, edge_load(buf_of(x), ct))
                 elif x.kind == 'gather':
                     src = x.args[0]
                     if self._rowlocal(x.obj, red):
@@ -564,7 +622,7 @@ class _Lowering:
                             emit_row_leaf(src)
                     else:
                         idx = self._param(kern, x.obj.idx, 'int')
-                        s = var(body, f'{buf_of(src)}[{idx}[e]]')
+                        s = var(body, f'{buf_of(src)}[{edge_load(idx, "int")}]')
                 elif x.kind == 'ew':
                     a = [emit_edge(y) for y in x.args]
                     s = var(body, ew_expr(x, a), 'bool' if is_pred(x) else None)
@@ -581,10 +639,30 @@ class _Lowering:
                 accs.append(f'acc{u.id}')
             inner = [emit_edge(u.args[0]) for u in members]
             rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
-            hp = red.easier_host_rowptr
-            counts = np.diff(hp)
-            kmax = int(counts.max()) if counts.size else 0
-            dense = 0 < kmax <= 8 and counts.mean() >= 0.5 * kmax
+            if stage:
+                cap = 256 * kmax
+                lds = sum(cap * (4 if c == 'int' else self.p.dtype.itemsize)
+                          for c in [c for _, c in staged.values()])
+                if kern.lds_bytes + lds > _LDS_BUDGET:
+                    raise _Restage()
+                kern.lds_bytes += lds
+                kern.staged = True
+                # the block's rows [rb, rb + 256) own the contiguous edge
+                # range [eb, eb + nl), nl <= 256 * kmax; copy it coalesced
+                for name, (sn, c) in staged.items():
+                    rows.append(f'  __shared__ {c} {sn}[{cap}];')
+                rows.append(f'  const int eb{gid} = {rp}[rb], nl{gid} = '
+                            f'{rp}[rb + 256 < a.n ? rb + 256 : a.n] - eb{gid};')
+                ld = (lambda nm, i: f'__builtin_nontemporal_load(&{nm}[{i}])') if nt else \
+                    (lambda nm, i: f'{nm}[{i}]')
+                rows.append(f'  for (int i = threadIdx.x; i < nl{gid}; i += 256) {{')
+                for name, (sn, c) in staged.items():
+                    rows.append(f'    {sn}[i] = {ld(name, f"eb{gid} + i")};')
+                rows.append('  }')
+                # an empty range still gives every slot a valid (row 0) entry
+                rows.append(f'  if (nl{gid} == 0 && threadIdx.x == 0) {{ ' +
+                            ' '.join(f'{sn}[0] = 0;' for sn, _ in staged.values()) + ' }')
+                rows.append('  __syncthreads();')
             if dense:
                 # every row has at most kmax entries: evaluate kmax slots with
                 # clamped (always valid) edge positions so all their loads
@@ -596,8 +674,12 @@ class _Lowering:
                 rows.append(f'  #pragma unroll')
                 rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
                 rows.append('    const int j = jb + k;')
-                rows.append('    const int jc = j < je ? j : (je > 0 ? je - 1 : 0);')
-                rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
+                if stage:
+                    rows.append(f'    int jl = j - eb{gid}; jl = jl < nl{gid} ? jl : nl{gid} - 1; '
+                                'jl = jl > 0 ? jl : 0;')
+                else:
+                    rows.append('    const int jc = j < je ? j : (je > 0 ? je - 1 : 0);')
+                    rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
                 rows.extend('  ' + b for b in body)
                 rows.append('  } }')
             else:
@@ -636,19 +718,32 @@ class _Lowering:
                     s = var(rows, f'{buf_of(val)}[r]')
                 else:
                     s = emit_row(val)
-                stores.append(f'  {self._param(kern, rt.obj.data, ct, const=False)}[r] = {s};')
+                stores.append((self._param(kern, rt.obj.data, ct, const=False), s))
             else:
                 s = emit_row(rt)
-                stores.append(f'  {self._param(kern, rt.buf, ct, const=False)}[r] = {s};')
+                stores.append((self._param(kern, rt.buf, ct, const=False), s))
+        if OPTIONS['nontemporal']:
+            st = [f'  __builtin_nontemporal_store({s}, &{d}[r]);' for d, s in stores]
+        else:
+            st = [f'  {d}[r] = {s};' for d, s in stores]
         params = ''.join(f'  {ctyp} __restrict__ p{i};\n'
                          for i, (_, ctyp, _) in enumerate(kern.params))
+        if kern.staged:
+            # every thread of the block takes part in the LDS staging: rows
+            # past the end recompute the last row and do not store
+            head = ('  const i64 rb = esr_remap(blockIdx.x, gridDim.x) * 256;\n'
+                    '  const bool live = rb + threadIdx.x < a.n;\n'
+                    '  const i64 r = live ? rb + threadIdx.x : a.n - 1;\n')
+            tail = '  if (!live) return;\n' + '\n'.join(st)
+        else:
+            head = ('  const i64 r = esr_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;\n'
+                    '  if (r >= a.n) return;\n')
+            tail = '\n'.join(st)
         kern.src = (
             f'struct Args_{kern.name} {{\n{params}  i64 n;\n}};\n'
             f'extern "C" __global__ __launch_bounds__(256) void {kern.name}'
-            f'(const Args_{kern.name} a) {{\n'
-            f'  const i64 r = esr_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;\n'
-            f'  if (r >= a.n) return;\n' + '\n'.join(rows) + '\n' +
-            '\n'.join(stores) + '\n}\n')
+            f'(const Args_{kern.name} a) {{\n' + head + '\n'.join(rows) + '\n' +
+            tail + '\n}\n')
 
     def pack(self, kern: Kernel) -> bytes:
         vals = [t.data_ptr() for (_, _, t) in kern.params]

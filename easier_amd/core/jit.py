@@ -99,6 +99,7 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
         if eng is None:
             eng = HipEngine(m, g, device)
         m.easier_engine = eng
+        m.easier_program, m.easier_graph = prog, g
         m.forward = eng.forward
     for m in prog.modules:
         hook = getattr(m, 'easier_after_compile', None)

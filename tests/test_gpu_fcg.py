@@ -1,0 +1,85 @@
+"""Flexible CG with the learned preconditioner: convergence on the true
+float64 residual, the Jacobi fallback, and that training lowers the loss
+below the untrained (Jacobi) network's."""
+import numpy as np
+import pytest
+import torch
+
+import easier_amd as esr
+from easier_amd.numeric.preconditioner.learned import LearnedPreconditioner
+from easier_amd.numeric.solver import CG, FlexibleCG
+from easier_amd.workloads import learned as wl
+from easier_amd.workloads import poisson as wp
+from oracle import gnn
+from oracle import solvers as osol
+
+pytestmark = pytest.mark.gpu
+
+
+def _program(n, seed, M):
+    mesh, sys_, kap = wp.varcoef_system(n, seed)
+    logk = wp.gaussian_random_field(n, seed).reshape(-1)
+    eqn = wp.PoissonP1(sys_)
+    sol = FlexibleCG(eqn.A, eqn.b, eqn.x, M,
+                     dict(points=mesh['points'], logk=logk, n_grid=n))
+    esr.compile([eqn, sol], 'hip')
+    return eqn, sol, sys_
+
+
+def _true_residual(sys_, x):
+    A = osol.Linsys(torch.from_numpy(sys_['Av']), torch.from_numpy(sys_['Ae']),
+                    torch.from_numpy(sys_['col']), torch.from_numpy(sys_['row']),
+                    sys_['Av'].shape[0])
+    return float((torch.from_numpy(sys_['b']) - A(x)).norm())
+
+
+def test_untrained_network_behaves_like_jacobi_pcg(gpu):
+    M = LearnedPreconditioner(3, gnn.init_params(3, 0))
+    eqn, sol, sys_ = _program(24, 1, M)
+    info = sol.solve(rtol=1e-8, maxiter=5000)
+    bn = float(np.linalg.norm(sys_['b']))
+    assert info['residual'] <= 1e-8 * bn and info['fallback_at'] is None
+    assert _true_residual(sys_, eqn.x.collect()) <= 1.01e-8 * bn
+    # against the fused Jacobi-PCG on the same system
+    eqn2 = wp.PoissonP1(sys_)
+    cg = CG(eqn2.A, eqn2.b, eqn2.x, eqn2.M)
+    esr.compile([eqn2, cg], 'hip')
+    info2 = cg.solve(rtol=1e-8, maxiter=5000)
+    assert abs(info['iters'] - (info2['iters'] + 1)) <= 3
+    x1, x2 = eqn.x.collect(), eqn2.x.collect()
+    assert float((x1 - x2).norm()) <= 1e-5 * float(x2.norm())
+
+
+def test_random_network_converges_and_fallback_path(gpu):
+    p = gnn.init_params(3, 4)
+    p['w_out'] = torch.randn(gnn.H, generator=torch.Generator().manual_seed(2)) * 0.5
+    M = LearnedPreconditioner(3, p)
+    eqn, sol, sys_ = _program(20, 2, M)
+    info = sol.solve(rtol=1e-8, maxiter=5000)
+    bn = float(np.linalg.norm(sys_['b']))
+    assert info['residual'] <= 1e-8 * bn
+    assert _true_residual(sys_, eqn.x.collect()) <= 1.01e-8 * bn
+    eqn.x.data.zero_()
+    info = sol.solve(rtol=1e-8, maxiter=5000, window=-1)   # stagnation at once
+    assert info['fallback_at'] == 0
+    assert info['residual'] <= 1e-8 * bn
+
+
+def test_training_beats_jacobi_on_the_training_loss(gpu):
+    probs = [wl.Problem(20, s) for s in (10, 11)]
+    data = [(pr, wl.generate_pairs(pr, n_rhs=4, seed=s)) for s, pr in enumerate(probs)]
+    assert sum(len(p) for _, p in data) >= 20
+    base = wl.jacobi_loss(data)
+    pre = LearnedPreconditioner(3, gnn.init_params(3, 0))
+    wl.train(pre, data, epochs=6, lr=2e-3)
+    pre_eval = LearnedPreconditioner(3, {k: v for k, v in pre.named().items()})
+    # loss of the trained network on the same pairs
+    tot, cnt = 0.0, 0
+    for prob, pairs in data:
+        pre_eval.bind(prob.graph)
+        for r, e, en2 in pairs:
+            z = pre_eval(r)
+            d = (z - e.to(torch.float64))
+            tot += float((d * d).sum()) / en2
+            cnt += 1
+    assert tot / cnt < 0.98 * base, (tot / cnt, base)
