@@ -55,7 +55,7 @@ class Unsupported(Exception):
 #                  edges from LDS;
 #   nontemporal  - edge-array loads and row stores carry the nontemporal hint
 #                  (streamed once, not kept in the caches).
-OPTIONS = {'stage_edges': False, 'nontemporal': False}
+OPTIONS = {'stage_edges': True, 'nontemporal': True}
 _LDS_BUDGET = 48 * 1024
 
 
