@@ -84,8 +84,8 @@ def time_kernels(eqn, reps):
     for _ in range(reps):
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(eng.launches) + 1)]
         evs[0].record(stream)
-        for k, (fn, grid, buf) in enumerate(eng.launches):
-            nat.call('esr_launch', fn, grid, 1, 1, 256, 1, 1, 0, sp,
+        for k, (fn, grid, buf, block) in enumerate(eng.launches):
+            nat.call('esr_launch', fn, grid, 1, 1, block, 1, 1, 0, sp,
                      ctypes.addressof(buf), ctypes.sizeof(buf))
             evs[k + 1].record(stream)
         torch.cuda.synchronize()
@@ -151,6 +151,167 @@ def poisson_config1(skip_cpu):
     return out
 
 
+def primitives(eqn, reps=20):
+    """The plain Selector / Reducer kernels on the compiled SWE mesh: the
+    cell -> edge gather of `gather_src` (esr_gather) and the edge -> cell CSR
+    sum of `scatter` (esr_csr_reduce), each timed with HIP events over `reps`
+    back-to-back launches on the launch stream; algorithmic bytes
+    n_out (I + T) + n_src T and n_in T + (n_out + 1) I + n_out T."""
+    import torch
+    from easier_amd.core import ops
+    stream = torch.cuda.current_stream()
+    src = eqn.h.data
+    sel, red = eqn.gather_src, eqn.scatter
+    T, I = src.element_size(), 4
+    edge_vals = ops.gather(src, sel.idx)
+    out_cells = torch.empty_like(src)
+
+    def run(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 1e3 / reps
+
+    t_g = run(lambda: ops.gather(src, sel.idx, out=edge_vals))
+    t_r = run(lambda: ops.csr_reduce(edge_vals, red.easier_rowptr, red.easier_perm,
+                                     red.easier_long_rows, red.n, 'sum', out=out_cells))
+    ne, nc = sel.idx.shape[0], src.shape[0]
+    bg = ne * (I + T) + nc * T
+    br = ne * T + (nc + 1) * I + nc * T
+    return {
+        'selector_gather': dict(edges=ne, seconds=t_g, bytes=bg, GBps=bg / t_g / 1e9,
+                                frac=bg / t_g / HBM_PEAK),
+        'reducer_csr_sum': dict(rows=nc, entries=ne, seconds=t_r, bytes=br,
+                                GBps=br / t_r / 1e9, frac=br / t_r / HBM_PEAK),
+    }
+
+
+def committed_traffic(n, dtype, kernel):
+    """HBM bytes per launch of `kernel` measured by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/ (lines 'traffic_bytes <kernel>
+    <bytes>'), or None when no summary covers this workload."""
+    path = os.path.join(REPO, 'profiles', f'swe_fused_{n}_{dtype}_traffic.md')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == 'traffic_bytes' and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
+def _best_time(fn, reps, reset=None):
+    import torch
+    best, out = None, None
+    for rep in range(reps + 1):                 # first call warms up
+        if reset is not None:
+            reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if rep > 0:
+            best = dt if best is None else min(best, dt)
+    return best, out
+
+
+def batched_config3(reps=1):
+    """(a) 64 right-hand sides U[-1,1] (seeds 0..63) on the 1024 x 1024 mesh;
+    (b) 256 meshes, n ~ U{32..512}, interior vertices jittered by U[-0.2h,
+    0.2h], f ~ U[-1,1] each: Jacobi-PCG FP64 to 1e-8 per instance, one batched
+    solve (autotuned tile height) against the instances solved one after the
+    other with the single-system fused solver."""
+    import numpy as np
+    from easier_amd.numeric.solver.batched import BatchedCG
+    from easier_amd.workloads import mesh as wmesh
+    from easier_amd.workloads import poisson as wp
+    out = {}
+    t0 = time.perf_counter()
+    m = wmesh.tri2(1024)
+    base = wp.p1_system(m, None, None)
+    nv = m['points'].shape[0]
+    sys_a = [dict(base, b=wp.p1_system(m, None, wp.rhs_uniform(nv, k))['b'])
+             for k in range(64)]
+    rng = np.random.default_rng(0)
+    sys_b = []
+    for k in range(256):
+        n = int(rng.integers(32, 513))
+        mk = wmesh.tri2(n, jitter=0.2, seed=k)
+        sys_b.append(wp.p1_system(mk, None, rng.uniform(-1, 1, mk['points'].shape[0])))
+    log(f'config 3 systems built ({time.perf_counter() - t0:.1f} s)')
+    for name, systems in (('3a_64rhs_1024', sys_a), ('3b_256meshes', sys_b)):
+        bcg = BatchedCG(systems)
+        R, timings = bcg.autotune(iters=5)
+        t, infos = _best_time(lambda: bcg.solve(rtol=1e-8, maxiter=20000), reps)
+        its = [i['iters'] for i in infos]
+        ok = all(i['residual'] <= 1e-8 * float(np.linalg.norm(s['b'])) * (1 + 1e-12)
+                 for i, s in zip(infos, systems))
+        ts = 0.0
+        for s in systems[:8]:                # one at a time, on a sample
+            single = BatchedCG([s])
+            single.R = R
+            dt, _ = _best_time(lambda: single.solve(rtol=1e-8, maxiter=20000), 1)
+            ts += dt
+        out[name] = dict(instances=len(systems), rows=int(bcg.offsets[-1]),
+                         tile_rows=256 * R, autotune_s=timings, solve_s=t,
+                         iters_max=max(its), iters_mean=float(np.mean(its)),
+                         all_converged=ok,
+                         sequential_first8_s=ts)
+        log(f'config {name}: {t:.3f} s batched, iters max {max(its)}')
+        del bcg
+    return out
+
+
+def mixed_config4(reps=1):
+    """Variable-coefficient Poisson on the 2048 x 2048 mesh (4,198,401
+    vertices): float64 Jacobi-PCG to 1e-10 against the mixed-precision solver
+    (float32 inner CG, float64 refinement) to 1e-10; error of the mixed
+    solution against the float64 one."""
+    import numpy as np
+    import easier_amd as esr
+    from easier_amd.numeric.solver import CG, MixedPrecisionCG
+    from easier_amd.workloads import poisson as wp
+    t0 = time.perf_counter()
+    mesh, sys_, _ = wp.varcoef_system(2048, seed=0)
+    log(f'config 4 system built ({time.perf_counter() - t0:.1f} s)')
+    e1 = wp.PoissonP1(sys_)
+    cg = CG(e1.A, e1.b, e1.x, e1.M)
+    esr.compile([e1, cg], 'hip')
+    t64, i64 = _best_time(lambda: cg.solve(rtol=1e-10, maxiter=100000), reps,
+                          reset=lambda: e1.x.data.zero_())
+    x64 = e1.x.collect().numpy()
+    e2 = wp.PoissonP1(sys_)
+    mp = MixedPrecisionCG(e2.A, e2.b, e2.x, e2.M)
+    esr.compile([e2, mp], 'hip')
+    tmx, imx = _best_time(lambda: mp.solve(rtol=1e-10, inner_rtol=1e-4, maxiter=100000),
+                          reps, reset=lambda: e2.x.data.zero_())
+    xmx = e2.x.collect().numpy()
+    out = dict(vertices=int(sys_['Av'].shape[0]), fp64_s=t64, fp64_iters=i64['iters'],
+               fp64_residual=i64['residual'], mixed_s=tmx, mixed_inner_iters=imx['iters'],
+               mixed_outer=imx['outer'], mixed_true_residual=imx['residual'],
+               bnorm=imx['bnorm'],
+               rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64)))
+    log(f'config 4: fp64 {t64:.3f} s ({i64["iters"]} it), mixed {tmx:.3f} s')
+    return out
+
+
+def learned_config5(weights, seeds=(1,), reps=1):
+    """Learned-preconditioner flexible CG against Jacobi-PCG on unseen
+    variable-coefficient problems on the 2048 x 2048 mesh, both to 1e-8."""
+    from easier_amd.numeric.preconditioner import learned as lp
+    from easier_amd.workloads import learned as wl
+    pre = lp.load(weights)
+    return [wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000)
+            for s in seeds]
+
+
 def copy_calibration(mib):
     """dst = src for a float64 array of `mib` MiB, 5 launches: reads and
     writes exactly mib MiB each; prints the event-timed bandwidth."""
@@ -188,6 +349,11 @@ def main():
     ap.add_argument('--no-fuse', action='store_true')
     ap.add_argument('--profile', action='store_true',
                     help='only warm-up + timed steps (for rocprofv3 runs)')
+    ap.add_argument('--configs', default='',
+                    help='comma list of extra workloads measured after the headline: '
+                         '3 (batched PCG), 4 (mixed precision), 5 (learned FCG)')
+    ap.add_argument('--weights', default=os.path.join(REPO, 'easier_amd', 'workloads',
+                                                      'gnn_poisson_l3_h32.npz'))
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy a float64 array of MIB MiB 5 times (the known-size '
                          'kernel that scales the HBM byte counters)')
@@ -260,12 +426,14 @@ def main():
         result['roofline'] = {
             'bound': 'hbm', 'kernel': f'esr_fused_{k} (RK4 stage {k + 1} of 4)',
             'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-            'frac': achieved / HBM_PEAK, 'traffic': None,
+            'frac': achieved / HBM_PEAK,
+            'traffic': committed_traffic(args.n, args.dtype, f'esr_fused_{k}'),
             'bytes_per_launch': stages[k], 'avg_launch_s': ktimes[k],
             'all_kernels_s': ktimes,
         }
         log(f'dominant kernel esr_fused_{k}: {ktimes[k] * 1e6:.1f} us, '
             f'{achieved / 1e9:.0f} GB/s ({achieved / HBM_PEAK:.1%} of 8 TB/s)')
+    result['primitives'] = primitives(eqn)
     if not args.skip_cpu:
         log(f'CPU baseline: {args.cpu_steps} oracle steps on the host')
         result['cpu_baseline'] = cpu_baseline_swe(mesh, geo, meta, dtype, args.cpu_steps)
@@ -273,6 +441,13 @@ def main():
     if not args.skip_poisson:
         log('Poisson config 1')
         result['poisson_config1'] = poisson_config1(args.skip_cpu)
+    extra = {c.strip() for c in args.configs.split(',') if c.strip()}
+    if '3' in extra:
+        result['batched_config3'] = batched_config3()
+    if '4' in extra:
+        result['mixed_config4'] = mixed_config4()
+    if '5' in extra:
+        result['learned_config5'] = learned_config5(args.weights)
     print(json.dumps(result), flush=True)
 
 

@@ -53,9 +53,17 @@ class Unsupported(Exception):
 #                  the block's contiguous edge range of every edge array it
 #                  reads into LDS with coalesced loads, then reads the rows'
 #                  edges from LDS;
-#   nontemporal  - edge-array loads and row stores carry the nontemporal hint
-#                  (streamed once, not kept in the caches).
-OPTIONS = {'stage_edges': True, 'nontemporal': True}
+#   nt_loads     - edge-array loads carry the nontemporal hint (streamed once);
+#   nt_stores    - row stores carry the nontemporal hint;
+#   block        - rows (threads) per workgroup;
+#   xcd_remap    - consecutive row tiles are placed on one XCD (its L2 then
+#                  holds the rows their neighbours gather); off, the 8 XCDs
+#                  stream through neighbouring rows together, which keeps the
+#                  ~20 concurrent array streams of a stage kernel on fewer
+#                  DRAM pages (measured faster for the SWE stage kernels).
+# Defaults are the fastest measured combination for the 2048 x 2048 SWE step.
+OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
+           'xcd_remap': False}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -383,6 +391,7 @@ class Kernel:
         self.src = ''
         self.staged = False
         self.lds_bytes = 0
+        self.block = int(OPTIONS['block'])
 
     def algorithmic_bytes(self) -> int:
         """Compulsory HBM bytes of one launch: every array the kernel reads
@@ -401,14 +410,14 @@ class FusedEngine:
 
     def __init__(self, module, launches, keepalive):
         self.module = module
-        self.launches = launches          # (fn, grid, argbytes)
+        self.launches = launches          # (fn, grid, argbytes, block)
         self.keepalive = keepalive
         self.graph = None
         self.calls = 0
 
     def _launch_all(self, stream):
-        for fn, grid, buf in self.launches:
-            nat.call('esr_launch', fn, grid, 1, 1, 256, 1, 1, 0, stream,
+        for fn, grid, buf, block in self.launches:
+            nat.call('esr_launch', fn, grid, 1, 1, block, 1, 1, 0, stream,
                      ctypes.addressof(buf), ctypes.sizeof(buf))
 
     def forward(self):
@@ -597,7 +606,8 @@ class _Lowering:
             # edge arrays staged through LDS: param name -> (lds name, ctype)
             staged: Dict[str, tuple] = {}
             stage = dense and perm is None and OPTIONS['stage_edges'] and not self.no_stage
-            nt = OPTIONS['nontemporal']
+            nt = OPTIONS['nt_loads']
+            B = kern.block
 
             def edge_load(name, ctype):
                 if stage:
@@ -640,22 +650,22 @@ class _Lowering:
             inner = [emit_edge(u.args[0]) for u in members]
             rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
             if stage:
-                cap = 256 * kmax
+                cap = B * kmax
                 lds = sum(cap * (4 if c == 'int' else self.p.dtype.itemsize)
                           for c in [c for _, c in staged.values()])
                 if kern.lds_bytes + lds > _LDS_BUDGET:
                     raise _Restage()
                 kern.lds_bytes += lds
                 kern.staged = True
-                # the block's rows [rb, rb + 256) own the contiguous edge
-                # range [eb, eb + nl), nl <= 256 * kmax; copy it coalesced
+                # the block's rows [rb, rb + B) own the contiguous edge
+                # range [eb, eb + nl), nl <= B * kmax; copy it coalesced
                 for name, (sn, c) in staged.items():
                     rows.append(f'  __shared__ {c} {sn}[{cap}];')
                 rows.append(f'  const int eb{gid} = {rp}[rb], nl{gid} = '
-                            f'{rp}[rb + 256 < a.n ? rb + 256 : a.n] - eb{gid};')
+                            f'{rp}[rb + {B} < a.n ? rb + {B} : a.n] - eb{gid};')
                 ld = (lambda nm, i: f'__builtin_nontemporal_load(&{nm}[{i}])') if nt else \
                     (lambda nm, i: f'{nm}[{i}]')
-                rows.append(f'  for (int i = threadIdx.x; i < nl{gid}; i += 256) {{')
+                rows.append(f'  for (int i = threadIdx.x; i < nl{gid}; i += {B}) {{')
                 for name, (sn, c) in staged.items():
                     rows.append(f'    {sn}[i] = {ld(name, f"eb{gid} + i")};')
                 rows.append('  }')
@@ -722,26 +732,28 @@ class _Lowering:
             else:
                 s = emit_row(rt)
                 stores.append((self._param(kern, rt.buf, ct, const=False), s))
-        if OPTIONS['nontemporal']:
+        if OPTIONS['nt_stores']:
             st = [f'  __builtin_nontemporal_store({s}, &{d}[r]);' for d, s in stores]
         else:
             st = [f'  {d}[r] = {s};' for d, s in stores]
         params = ''.join(f'  {ctyp} __restrict__ p{i};\n'
                          for i, (_, ctyp, _) in enumerate(kern.params))
+        tile = 'esr_remap(blockIdx.x, gridDim.x)' if OPTIONS['xcd_remap'] else \
+            '(i64)blockIdx.x'
         if kern.staged:
             # every thread of the block takes part in the LDS staging: rows
             # past the end recompute the last row and do not store
-            head = ('  const i64 rb = esr_remap(blockIdx.x, gridDim.x) * 256;\n'
+            head = (f'  const i64 rb = {tile} * {kern.block};\n'
                     '  const bool live = rb + threadIdx.x < a.n;\n'
                     '  const i64 r = live ? rb + threadIdx.x : a.n - 1;\n')
             tail = '  if (!live) return;\n' + '\n'.join(st)
         else:
-            head = ('  const i64 r = esr_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;\n'
+            head = (f'  const i64 r = {tile} * {kern.block} + threadIdx.x;\n'
                     '  if (r >= a.n) return;\n')
             tail = '\n'.join(st)
         kern.src = (
             f'struct Args_{kern.name} {{\n{params}  i64 n;\n}};\n'
-            f'extern "C" __global__ __launch_bounds__(256) void {kern.name}'
+            f'extern "C" __global__ __launch_bounds__({kern.block}) void {kern.name}'
             f'(const Args_{kern.name} a) {{\n' + head + '\n'.join(rows) + '\n' +
             tail + '\n}\n')
 
@@ -794,8 +806,8 @@ def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
         nat.call('esr_rtc_function', mod, k.name.encode(), ctypes.byref(fn))
         raw = low.pack(k)
         buf = (ctypes.c_char * len(raw)).from_buffer_copy(raw)
-        grid = max(1, (k.n + 255) // 256)
-        launches.append((fn, grid, buf))
+        grid = max(1, (k.n + k.block - 1) // k.block)
+        launches.append((fn, grid, buf, k.block))
     keep = [v.buf for v in p.values if v.buf is not None]
     eng = FusedEngine(m, launches, keep)
     eng.source = src

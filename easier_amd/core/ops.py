@@ -231,9 +231,21 @@ _AGG = {'sum': nat.AGG_SUM, 'prod': nat.AGG_PROD, 'norm': nat.AGG_NORM,
         'max': nat.AGG_MAX, 'min': nat.AGG_MIN}
 
 
-def aggregate(op: str, t: torch.Tensor, p=2.0, keep_leading=True):
+def aggregate_ws_bytes(t: torch.Tensor) -> int:
+    """Bytes of the workspace esr_aggregate needs for `t`."""
+    width = 1
+    for s in t.shape[1:]:
+        width *= s
+    nb = ctypes.c_int64(0)
+    nat.call('esr_aggregate_ws_bytes', nat.dtype_code(t.dtype),
+             t.shape[0] if t.dim() > 0 else 1, width, ctypes.byref(nb))
+    return max(nb.value, 8)
+
+
+def aggregate(op: str, t: torch.Tensor, p=2.0, keep_leading=True, out=None, ws=None):
     """torch.<op>(t, dim=0, keepdim=True) (norm: p-norm) as a deterministic
-    two-stage device reduction."""
+    two-stage device reduction. With `out` and `ws` (aggregate_ws_bytes
+    bytes) given, nothing is allocated (safe inside a captured graph)."""
     _dev_check(t)
     if op == 'norm':
         if isinstance(p, str):
@@ -247,13 +259,16 @@ def aggregate(op: str, t: torch.Tensor, p=2.0, keep_leading=True):
     for s in t.shape[1:]:
         width *= s
     out_shape = ((1,) + tuple(t.shape[1:])) if keep_leading else tuple(t.shape[1:])
-    out = empty(out_shape, t.dtype, t.device)
+    if out is None:
+        out = empty(out_shape, t.dtype, t.device)
+    elif out.numel() != max(width, 1) or out.dtype != t.dtype or not out.is_contiguous():
+        raise ValueError('aggregate: `out` must be a contiguous tensor of the result size')
     if width == 0:
         return out
-    nb = ctypes.c_int64(0)
-    nat.call('esr_aggregate_ws_bytes', nat.dtype_code(t.dtype), n, width,
-             ctypes.byref(nb))
-    ws = torch.empty((max(nb.value, 8),), dtype=torch.uint8, device=t.device)
+    if ws is None:
+        ws = torch.empty((aggregate_ws_bytes(t),), dtype=torch.uint8, device=t.device)
+    elif ws.numel() * ws.element_size() < aggregate_ws_bytes(t):
+        raise ValueError('aggregate: workspace too small')
     nat.call('esr_aggregate', _AGG[op], nat.dtype_code(t.dtype), t.data_ptr(),
              n, width, float(p), out.data_ptr(), ws.data_ptr(), ws.numel(),
              nat.stream_ptr())

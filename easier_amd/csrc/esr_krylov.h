@@ -1,0 +1,116 @@
+// Device helpers shared by the Krylov-solver kernels: grid-stride row loops,
+// the Linsys row product, and fence-free deterministic grid reductions.
+//
+// Reductions. The MI355X's L2 caches are per XCD and not coherent with each
+// other, so an agent-scope release fence -- what a "the last workgroup to
+// arrive finishes the sum" reduction needs before its arrival atomic --
+// writes back and invalidates the XCD's whole L2 (buffer_wbl2 / buffer_inv),
+// once per workgroup. Here a reduction kernel instead stores one partial per
+// workgroup with plain stores, and the next kernel on the stream (ordered
+// after every store of the previous one) sums all partials in every
+// workgroup, in one fixed order, so every workgroup holds the same bits. The
+// grid of such kernels is capped at kMaxGrid workgroups (rows are visited
+// with a grid-stride loop) so that re-summing the partials stays cheap.
+#pragma once
+#include "esr_common.h"
+
+namespace esr {
+
+constexpr int kRB = 256;        // threads per workgroup of the solver kernels
+constexpr int kMaxGrid = 1024;  // workgroups of a grid-stride solver kernel
+
+inline unsigned red_grid(int64_t n) {
+  const int64_t g = ceil_div(n, kRB);
+  return (unsigned)(g < kMaxGrid ? (g > 0 ? g : 1) : kMaxGrid);
+}
+
+// Rows of a grid-stride kernel: i = blockIdx.x * kRB + threadIdx.x + k * stride.
+#define ESR_GRID_ROWS(i, n)                                                    \
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < (n);           \
+       i += (int64_t)gridDim.x * kRB)
+
+__device__ __forceinline__ double kr_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Workgroup sums of NV per-thread values, written to part[k * G + blockIdx.x]
+// (G = gridDim.x) by thread 0. Fixed combination order.
+template <int NV>
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double *part) {
+  __shared__ double sh[NV][kRB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = kr_wave_sum(v[k]);
+    if (lane == 0) sh[k][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kRB / 64; ++w) s += sh[k][w];
+      part[(int64_t)k * gridDim.x + blockIdx.x] = s;
+    }
+  }
+  __syncthreads();  // sh may be reused by a following reduction
+}
+
+// out[k] = sum_{i < cnt} part[k * stride + first + i] for every thread of the
+// workgroup, in a fixed order (identical in every workgroup).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int64_t stride, int64_t first,
+                                             int64_t cnt, double (&out)[NV]) {
+  __shared__ double sh[NV][kRB / 64];
+  __shared__ double res[NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < cnt; i += kRB) s += part[k * stride + first + i];
+    s = kr_wave_sum(s);
+    if (lane == 0) sh[k][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+      for (int w = 0; w < kRB / 64; ++w) t += sh[k][w];
+      res[k] = t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = res[k];
+  __syncthreads();  // res / sh may be reused by the caller's next reduction
+}
+
+// Linsys row (easier/numeric/linsys.py:23-24): the off-diagonal CSR sum of
+// row r folded from 0 in entry order, plus the diagonal term. The first 8
+// entries are evaluated as independent slots (clamped positions, so every
+// load is valid and all of them are in flight together) and folded in only
+// when they exist: the same left-to-right order as the plain loop.
+template <typename T, typename Src>
+__device__ __forceinline__ T linsys_row(int64_t r, const int32_t *__restrict__ rowptr,
+                                        const int32_t *__restrict__ col,
+                                        const T *__restrict__ Ae, const T *__restrict__ Av,
+                                        Src src) {
+  T s = T(0);
+  const int32_t b = rowptr[r], e = rowptr[r + 1];
+  if (e > b) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t j = b + k;
+      const int32_t jc = j < e ? j : e - 1;
+      const T v = src(col[jc]) * Ae[jc];
+      s = j < e ? s + v : s;
+    }
+    for (int32_t j = b + 8; j < e; ++j) s = s + src(col[j]) * Ae[j];
+  }
+  return s + src(r) * Av[r];
+}
+
+} // namespace esr

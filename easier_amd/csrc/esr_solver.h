@@ -26,7 +26,6 @@ struct PcgState {
   int64_t done;    // 1 once the loop has exited
   int64_t iters;   // value of `iters` at exit
   int64_t pad0;
-  uint32_t counter[4]; // last-block-done counters (zero between launches)
 };
 
 // Restarted GMRES(m) Arnoldi cycle (easier/numeric/solver/gmres.py:214-266).
@@ -36,7 +35,6 @@ struct GmresState {
   double H[33 * 32];  // (m+1) x m, row-major with row stride 32 (m <= 32)
   int64_t broken;     // Arnoldi breakdown (h < 1e-15) seen at step jend
   int64_t jend;       // last Arnoldi step of the cycle
-  uint32_t counter[4];
 };
 
 constexpr int kGmresMaxRestart = 32;

@@ -6,9 +6,12 @@
 //   q = A p; alpha = (r.z) / (p.q); r_old = r; r -= alpha q; x += alpha p
 //   z = M(r); beta = z.(r - r_old) / (r_old.z_old); p = z + beta p
 //
-// Scalars stay on the device (FcgState); dot products are deterministic
-// (last-workgroup reductions, float64).
+// Scalars stay on the device (FcgState). Each reducing operation is a
+// grid-stride kernel storing one partial per workgroup and a one-workgroup
+// kernel summing them in a fixed order (esr_krylov.h): deterministic, no
+// atomics, no fences.
 #include "esr_common.h"
+#include "esr_krylov.h"
 
 namespace esr {
 struct FcgState {
@@ -21,103 +24,42 @@ struct FcgState {
   double beta;
   double rr_true; // ||b - A x||^2
   double bb;
-  uint32_t counter[4];
+  double tol2;    // the loop exits once rr <= tol2 ...
+  int64_t maxit;  // ... or after maxit steps (< 0: no limit) ...
+  int64_t it;     // ... steps taken since esr_fcg_set
+  int64_t done;   // 1: the step operations are no-ops
+  int64_t nbad;   // directions with z.r <= 0 or not finite since esr_fcg_set
 };
 
 namespace {
-constexpr int kB = 256, kR = 4, kT = kB * kR;
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-template <int NV>
-__device__ bool sum_last(double (&v)[NV], double *part, uint32_t *counter, int64_t nb, int64_t b,
-                         double (&out)[NV]) {
-  __shared__ double sh[NV][kB / 64];
-  __shared__ int last;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const double s = wsum(v[k]);
-    if (lane == 0) sh[k][wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kB / 64; ++w) s += sh[k][w];
-      __hip_atomic_store(part + k * nb + b, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    last = (atomicAdd(counter, 1u) == (uint32_t)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < nb; i += kB)
-      s += __hip_atomic_load(part + k * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = wsum(s);
-    __syncthreads();
-    if (lane == 0) sh[k][wid] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kB / 64; ++w) t += sh[k][w];
-      out[k] = t;
-    }
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
-template <typename Src>
-__device__ __forceinline__ double spmv(int64_t r, const int32_t *__restrict__ rowptr,
-                                       const int32_t *__restrict__ col,
-                                       const double *__restrict__ Ae,
-                                       const double *__restrict__ Av, Src src) {
-  double s = 0.0;
-  for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) s = s + src(col[j]) * Ae[j];
-  return s + src(r) * Av[r];
-}
-
-#define ROWS_OF_BLOCK                                                          \
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);                          \
-  for (int k_ = 0; k_ < kR; ++k_) {                                            \
-    const int64_t i = b * kT + k_ * kB + threadIdx.x;                          \
-    if (i >= n) break;
-
-__global__ __launch_bounds__(kB) void k_fcg_spmv(int64_t n, const int32_t *rowptr, const int32_t *col,
-                                                const double *Ae, const double *Av,
-                                                const double *__restrict__ p, double *__restrict__ q,
-                                                FcgState *st, double *part) {
+__global__ __launch_bounds__(kRB) void k_fcg_spmv(int64_t n, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ col,
+                                                 const double *__restrict__ Ae,
+                                                 const double *__restrict__ Av,
+                                                 const double *__restrict__ p,
+                                                 double *__restrict__ q, const FcgState *st,
+                                                 double *part) {
+  if (st->done) return;
   double acc[1] = {0.0};
-  ROWS_OF_BLOCK
-    const double y = spmv(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
+  ESR_GRID_ROWS(i, n) {
+    const double y = linsys_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
     q[i] = y;
     acc[0] += p[i] * y;
   }
-  double tot[1];
-  if (sum_last<1>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->pq = tot[0];
-    st->alpha = st->rz / tot[0];
-  }
+  store_partials<1>(acc, part);
 }
 
-__global__ __launch_bounds__(kB) void k_fcg_update(int64_t n, double *__restrict__ x,
-                                                  double *__restrict__ r, double *__restrict__ ro,
-                                                  const double *__restrict__ p,
-                                                  const double *__restrict__ q, FcgState *st,
-                                                  double *part) {
+__global__ __launch_bounds__(kRB) void k_fcg_update(int64_t n, double *__restrict__ x,
+                                                   double *__restrict__ r,
+                                                   double *__restrict__ ro,
+                                                   const double *__restrict__ p,
+                                                   const double *__restrict__ q,
+                                                   const FcgState *st, double *part) {
+  if (st->done) return;
   const double alpha = st->alpha;
   double acc[1] = {0.0};
-  ROWS_OF_BLOCK
+  ESR_GRID_ROWS(i, n) {
     const double ri = r[i];
     ro[i] = ri;
     const double rn = ri - alpha * q[i];
@@ -125,80 +67,108 @@ __global__ __launch_bounds__(kB) void k_fcg_update(int64_t n, double *__restrict
     x[i] = x[i] + alpha * p[i];
     acc[0] += rn * rn;
   }
-  double tot[1];
-  if (sum_last<1>(acc, part, &st->counter[1], gridDim.x, b, tot) && threadIdx.x == 0)
-    st->rr = tot[0];
+  store_partials<1>(acc, part);
 }
 
-// zr = z.r, zro = z.r_old; beta = (zr - zro) / rz; then rz <- zr
-__global__ __launch_bounds__(kB) void k_fcg_dots(int64_t n, const double *__restrict__ z,
-                                                const double *__restrict__ r,
-                                                const double *__restrict__ ro, int restart,
-                                                FcgState *st, double *part) {
+__global__ __launch_bounds__(kRB) void k_fcg_dots(int64_t n, const double *__restrict__ z,
+                                                 const double *__restrict__ r,
+                                                 const double *__restrict__ ro,
+                                                 const FcgState *st, double *part) {
+  if (st->done) return;
   double acc[2] = {0.0, 0.0};
-  ROWS_OF_BLOCK
+  ESR_GRID_ROWS(i, n) {
     acc[0] += z[i] * r[i];
     acc[1] += z[i] * ro[i];
   }
-  double tot[2];
-  if (sum_last<2>(acc, part, &st->counter[2], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->zr = tot[0];
-    st->zro = tot[1];
-    st->beta = restart ? 0.0 : (tot[0] - tot[1]) / st->rz;
-    st->rz = tot[0];
-  }
+  store_partials<2>(acc, part);
 }
 
-__global__ __launch_bounds__(kB) void k_fcg_dir(int64_t n, const double *__restrict__ z,
-                                               double *__restrict__ p, const FcgState *st) {
+__global__ __launch_bounds__(kRB) void k_fcg_dir(int64_t n, const double *__restrict__ z,
+                                                double *__restrict__ p, const FcgState *st) {
+  if (st->done) return;
   const double beta = st->beta;
-  ROWS_OF_BLOCK
-    p[i] = z[i] + beta * p[i];
-  }
+  ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
 }
 
-// r = b - A x (float64) and ||r||^2, ||b||^2
-__global__ __launch_bounds__(kB) void k_fcg_residual(int64_t n, const int32_t *rowptr,
-                                                    const int32_t *col, const double *Ae,
-                                                    const double *Av, const double *__restrict__ bv,
-                                                    const double *__restrict__ x,
-                                                    double *__restrict__ r, FcgState *st,
-                                                    double *part) {
+// r = b - A x (float64); partials of ||r||^2 and ||b||^2
+__global__ __launch_bounds__(kRB) void k_fcg_residual(
+    int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const double *__restrict__ Ae, const double *__restrict__ Av, const double *__restrict__ bv,
+    const double *__restrict__ x, double *__restrict__ r, double *part) {
   double acc[2] = {0.0, 0.0};
-  ROWS_OF_BLOCK
-    const double ri = bv[i] - spmv(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
+  ESR_GRID_ROWS(i, n) {
+    const double ri = bv[i] - linsys_row<double>(i, rowptr, col, Ae, Av,
+                                                 [&](int64_t c) { return x[c]; });
     r[i] = ri;
     acc[0] += ri * ri;
     acc[1] += bv[i] * bv[i];
   }
-  double tot[2];
-  if (sum_last<2>(acc, part, &st->counter[3], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->rr_true = tot[0];
-    st->rr = tot[0];
-    st->bb = tot[1];
-  }
+  store_partials<2>(acc, part);
 }
 
 // z = r / d (the Jacobi fallback)
-__global__ __launch_bounds__(kB) void k_fcg_jacobi(int64_t n, const double *__restrict__ r,
-                                                  const double *__restrict__ d,
-                                                  double *__restrict__ z) {
-  ROWS_OF_BLOCK
-    z[i] = r[i] / d[i];
+__global__ __launch_bounds__(kRB) void k_fcg_jacobi(int64_t n, const double *__restrict__ r,
+                                                   const double *__restrict__ d,
+                                                   double *__restrict__ z, const FcgState *st) {
+  if (st->done) return;
+  ESR_GRID_ROWS(i, n) z[i] = r[i] / d[i];
+}
+
+// The scalars of a reducing operation, from its partials (one workgroup):
+// spmv: pq, alpha = rz / pq; update: rr, one more step, exit test;
+// dots: zr, zro, beta, rz <- zr (beta = 0 on a restart), non-descent count;
+// residual: rr_true = rr, bb. The step operations do nothing once `done`.
+__global__ __launch_bounds__(kRB) void k_fcg_fin(int op, int64_t g, const double *part,
+                                                FcgState *st, int restart) {
+  if (op != 4 && st->done) return;
+  double tot[2] = {0.0, 0.0};
+  if (op == 0 || op == 1) {
+    double one[1];
+    sum_partials<1>(part, g, 0, g, one);
+    tot[0] = one[0];
+  } else {
+    sum_partials<2>(part, g, 0, g, tot);
+  }
+  if (threadIdx.x != 0) return;
+  switch (op) {
+  case 0:
+    st->pq = tot[0];
+    st->alpha = st->rz / tot[0];
+    break;
+  case 1: {
+    st->rr = tot[0];
+    const int64_t it = st->it + 1;
+    st->it = it;
+    if (!(tot[0] > st->tol2) || (st->maxit >= 0 && it >= st->maxit)) st->done = 1;
+    break;
+  }
+  case 2:
+    st->zr = tot[0];
+    st->zro = tot[1];
+    st->beta = restart ? 0.0 : (tot[0] - tot[1]) / st->rz;
+    st->rz = tot[0];
+    if (!(tot[0] > 0.0) || !isfinite(tot[0])) st->nbad += 1;
+    break;
+  case 4:
+    st->rr_true = tot[0];
+    st->rr = tot[0];
+    st->bb = tot[1];
+    break;
+  default:
+    break;
   }
 }
-#undef ROWS_OF_BLOCK
 
-unsigned grid(int64_t n) { return (unsigned)ceil_div(n, kT); }
 } // namespace
 } // namespace esr
 
 using namespace esr;
 
+// Partials: up to two values per workgroup, G = red_grid(n) workgroups.
 ESR_API int esr_fcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_fcg_workspace: bad args");
   *state_bytes = (int64_t)sizeof(FcgState);
-  *ws_bytes = 2 * (int64_t)grid(n) * (int64_t)sizeof(double);
+  *ws_bytes = 2 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -213,41 +183,46 @@ ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *
   hipStream_t s = as_stream(stream);
   FcgState *st = (FcgState *)state;
   double *part = (double *)ws;
+  const unsigned g = red_grid(n);
   switch (op) {
   case 0:
     ESR_REQUIRE(rowptr && col && Ae && Av && p && q, "esr_fcg_op spmv: null");
-    k_fcg_spmv<<<grid(n), kB, 0, s>>>(n, rowptr, col, Ae, Av, p, q, st, part);
+    k_fcg_spmv<<<g, kRB, 0, s>>>(n, rowptr, col, Ae, Av, p, q, st, part);
     break;
   case 1:
     ESR_REQUIRE(x && r && ro && p && q, "esr_fcg_op update: null");
-    k_fcg_update<<<grid(n), kB, 0, s>>>(n, x, r, ro, p, q, st, part);
+    k_fcg_update<<<g, kRB, 0, s>>>(n, x, r, ro, p, q, st, part);
     break;
   case 2:
     ESR_REQUIRE(z && r && ro, "esr_fcg_op dots: null");
-    k_fcg_dots<<<grid(n), kB, 0, s>>>(n, z, r, ro, a0, st, part);
+    k_fcg_dots<<<g, kRB, 0, s>>>(n, z, r, ro, st, part);
     break;
   case 3:
     ESR_REQUIRE(z && p, "esr_fcg_op direction: null");
-    k_fcg_dir<<<grid(n), kB, 0, s>>>(n, z, p, st);
+    k_fcg_dir<<<g, kRB, 0, s>>>(n, z, p, st);
     break;
   case 4:
     ESR_REQUIRE(rowptr && col && Ae && Av && b && x && r, "esr_fcg_op residual: null");
-    k_fcg_residual<<<grid(n), kB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r, st, part);
+    k_fcg_residual<<<g, kRB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r, part);
     break;
   case 5:
     ESR_REQUIRE(r && d && z, "esr_fcg_op jacobi: null");
-    k_fcg_jacobi<<<grid(n), kB, 0, s>>>(n, r, d, z);
+    k_fcg_jacobi<<<g, kRB, 0, s>>>(n, r, d, z, st);
     break;
   default:
     set_error("esr_fcg_op: unknown op %d", op);
     return ESR_ERR_INVALID;
   }
+  if (op == 0 || op == 1 || op == 2 || op == 4)
+    k_fcg_fin<<<1, kRB, 0, s>>>(op, g, part, st, a0);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
 }
 
-ESR_API int esr_fcg_read(const void *state, double *scalars, void *stream) {
+// scalars = {rz, pq, alpha, rr, zr, zro, beta, rr_true, bb};
+// ints (nullable) = {it, done, nbad}.
+ESR_API int esr_fcg_read(const void *state, double *scalars, int64_t *ints, void *stream) {
   ESR_REQUIRE(state && scalars, "esr_fcg_read: null");
   FcgState h;
   hipStream_t s = as_stream(stream);
@@ -255,5 +230,31 @@ ESR_API int esr_fcg_read(const void *state, double *scalars, void *stream) {
   ESR_HIP(hipStreamSynchronize(s));
   const double v[9] = {h.rz, h.pq, h.alpha, h.rr, h.zr, h.zro, h.beta, h.rr_true, h.bb};
   for (int k = 0; k < 9; ++k) scalars[k] = v[k];
+  if (ints) {
+    ints[0] = h.it;
+    ints[1] = h.done;
+    ints[2] = h.nbad;
+  }
   return ESR_OK;
+}
+
+namespace {
+__global__ void k_fcg_set(FcgState *st, double tol2, int64_t maxit) {
+  st->tol2 = tol2;
+  st->maxit = maxit;
+  st->it = 0;
+  st->done = 0;
+  st->nbad = 0;
+}
+} // namespace
+
+// Arms the step operations: exit once rr <= tol2 or after maxit steps
+// (< 0: no limit); clears the step and non-descent counters.
+ESR_API int esr_fcg_set(void *state, double tol2, int64_t maxit, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(state, "esr_fcg_set: null state");
+  k_fcg_set<<<1, 1, 0, as_stream(stream)>>>((FcgState *)state, tol2, maxit);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
 }

@@ -5,102 +5,32 @@
 // with the row sum folded from 0 left to right, which is what the reference's
 // Reducer computes for the renumbered, row-sorted edge list.
 //
-// Dot products are accumulated in double with a fixed tree and finalised by
-// the last workgroup to finish (an integer arrival counter, no float atomics),
-// so every run produces the same bits. Solver scalars stay on the device.
+// Dot products are accumulated in double, one partial per workgroup, and
+// summed in a fixed order by every workgroup of the kernel that consumes
+// them (esr_krylov.h): no atomics, no fences, the same bits on every run.
+// Solver scalars stay on the device; every kernel of an iteration is a
+// no-op once the loop has exited, so chunks of iterations can be captured
+// in a HIP graph and replayed until the host sees `done`.
 #include "esr_common.h"
+#include "esr_krylov.h"
 #include "esr_solver.h"
 
 namespace esr {
 namespace {
 
-constexpr int kB = 256;          // threads per workgroup
-constexpr int kRowsPerThread = 4;
-constexpr int kTile = kB * kRowsPerThread;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Sums NV per-thread values over the workgroup, stores the NV workgroup sums
-// at part[k * nb + b]; the last workgroup to arrive sums part[k * nb + 0..nb)
-// in index order and returns true with the totals in out[] (thread 0).
-template <int NV>
-__device__ bool grid_sum_last(double (&v)[NV], double *__restrict__ part,
-                              uint32_t *counter, int64_t nb, int64_t b,
-                              double (&out)[NV]) {
-  __shared__ double sh[NV][kB / 64];
-  __shared__ int last;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum(v[k]);
-    if (lane == 0) sh[k][wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kB / 64; ++w) s += sh[k][w];
-      __hip_atomic_store(part + k * nb + b, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    const uint32_t prev = atomicAdd(counter, 1u);
-    last = (prev == (uint32_t)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < nb; i += kB)
-      s += __hip_atomic_load(part + k * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = wave_sum(s);
-    __syncthreads();
-    if (lane == 0) sh[k][wid] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kB / 64; ++w) t += sh[k][w];
-      out[k] = t;
-    }
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
-template <typename T, typename Src>
-__device__ __forceinline__ T spmv_row(int64_t r, const int32_t *__restrict__ rowptr,
-                                      const int32_t *__restrict__ col,
-                                      const T *__restrict__ Ae, const T *__restrict__ Av,
-                                      Src src) {
-  T s = T(0);
-  const int32_t b = rowptr[r], e = rowptr[r + 1];
-  for (int32_t j = b; j < e; ++j) s = s + src(col[j]) * Ae[j];
-  return s + src(r) * Av[r];
-}
-
 // ---------------------------------------------------------------------------
 // PCG
 // ---------------------------------------------------------------------------
-// Init (cg.py:25-29): r = b - A x; z = M r; p = z; bnorm = ||b||; rz = r.z;
-// tol = max(rtol*bnorm, atol) if atol else rtol*bnorm (cg.py:112-115).
+// Init (cg.py:25-29): r = b - A x; z = M r; p = z; partials of r.z and b.b.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_init(
+__global__ __launch_bounds__(kRB) void k_pcg_init(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ diag,
     const T *__restrict__ bvec, const T *__restrict__ x, T *__restrict__ r,
-    T *__restrict__ z, T *__restrict__ p, PcgState *st, double *part) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    T *__restrict__ z, T *__restrict__ p, double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
-    const T ax = spmv_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
+  ESR_GRID_ROWS(i, n) {
+    const T ax = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
     const T ri = bvec[i] - ax;
     const T zi = diag ? ri / diag[i] : ri;
     r[i] = ri;
@@ -109,8 +39,16 @@ __global__ __launch_bounds__(kB) void k_pcg_init(
     acc[0] += (double)(ri * zi);
     acc[1] += (double)(bvec[i] * bvec[i]);
   }
+  store_partials<2>(acc, part);
+}
+
+// bnorm = ||b||; tol = max(rtol*bnorm, atol) if atol else rtol*bnorm
+// (cg.py:112-115); loop counters reset. One workgroup.
+__global__ __launch_bounds__(kRB) void k_pcg_init_fin(int64_t g, const double *__restrict__ part,
+                                                     PcgState *st) {
   double tot[2];
-  if (grid_sum_last<2>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
+  sum_partials<2>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) {
     st->rz_cur = tot[0];
     st->rz_next = tot[0];
     st->bnorm = cr_sqrt(tot[1]);
@@ -124,48 +62,48 @@ __global__ __launch_bounds__(kB) void k_pcg_init(
   }
 }
 
-// Step part 1 (cg.py:44-47): Ap = A p; pAp = p.Ap; alpha = rz / pAp.
+// Step part 1 (cg.py:44-47): Ap = A p; partials of p.Ap. Workgroup 0 also
+// commits the counters the previous Update produced (nobody else reads them
+// in this kernel).
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_spmv(
+__global__ __launch_bounds__(kRB) void k_pcg_spmv(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ p,
-    T *__restrict__ Ap, PcgState *st, double *part) {
+    T *__restrict__ Ap, PcgState *st, double *__restrict__ part) {
   if (st->done) return;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->k_cur = st->k_next;   // nobody reads these in this kernel
+    st->k_cur = st->k_next;
     st->rz_cur = st->rz_next;
   }
   double acc[1] = {0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
-    const T y = spmv_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
+  ESR_GRID_ROWS(i, n) {
+    const T y = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
     Ap[i] = y;
     acc[0] += (double)(p[i] * y);
   }
-  double tot[1];
-  if (grid_sum_last<1>(acc, part, &st->counter[1], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->pAp = tot[0];
-    st->alpha = st->rz_next / tot[0];
-  }
+  store_partials<1>(acc, part);
 }
 
-// Step part 2 (cg.py:48-52) fused with the first half of Update (cg.py:67):
-// x += alpha p; r -= alpha Ap; z = M r; rr = r.r; rz = r.z.
+// Step part 2 (cg.py:47-52) fused with the first half of Update (cg.py:67):
+// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M r; partials of r.r, r.z.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_xr(int64_t n, const T *__restrict__ diag,
-                                              T *__restrict__ x, T *__restrict__ r,
-                                              T *__restrict__ z, const T *__restrict__ p,
-                                              const T *__restrict__ Ap, PcgState *st,
-                                              double *part) {
+__global__ __launch_bounds__(kRB) void k_pcg_xr(int64_t n, const T *__restrict__ diag,
+                                               T *__restrict__ x, T *__restrict__ r,
+                                               T *__restrict__ z, const T *__restrict__ p,
+                                               const T *__restrict__ Ap, PcgState *st,
+                                               const double *__restrict__ part_in,
+                                               double *__restrict__ part_out) {
   if (st->done) return;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const T alpha = (T)st->alpha;
+  double pAp[1];
+  sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, pAp);
+  const double alpha_d = st->rz_cur / pAp[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pAp = pAp[0];
+    st->alpha = alpha_d;
+  }
+  const T alpha = (T)alpha_d;
   double acc[2] = {0.0, 0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
+  ESR_GRID_ROWS(i, n) {
     const T pi = p[i];
     const T ri = r[i] - alpha * Ap[i];
     x[i] = x[i] + alpha * pi;
@@ -175,41 +113,43 @@ __global__ __launch_bounds__(kB) void k_pcg_xr(int64_t n, const T *__restrict__ 
     acc[0] += (double)(ri * ri);
     acc[1] += (double)(ri * zi);
   }
-  double tot[2];
-  if (grid_sum_last<2>(acc, part, &st->counter[2], gridDim.x, b, tot) && threadIdx.x == 0) {
-    st->rr = tot[0];
-    st->rz_new = tot[1];
-    st->rnorm = cr_sqrt(tot[0]);
-  }
+  store_partials<2>(acc, part_out);
 }
 
-// Loop test (cg.py:126-129) and Update (cg.py:67-71): p = z + beta p.
+// Loop test (cg.py:126-129) and Update (cg.py:67-71): rnorm = ||r||;
+// stop, or beta = rz_new / rz and p = z + beta p. Once the loop has exited
+// the two kernels before this one do nothing, and this one re-derives the
+// same exit from the unchanged partials (idempotent).
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_p(int64_t n, const T *__restrict__ z,
-                                             T *__restrict__ p, PcgState *st) {
-  if (st->done) return;
-  const double rnorm = st->rnorm;
+__global__ __launch_bounds__(kRB) void k_pcg_p(int64_t n, const T *__restrict__ z,
+                                              T *__restrict__ p, PcgState *st,
+                                              const double *__restrict__ part_in) {
+  double s[2];
+  sum_partials<2>(part_in, gridDim.x, 0, gridDim.x, s);
+  const double rnorm = cr_sqrt(s[0]);
   const int64_t k = st->k_cur;
   const bool converged = (rnorm == rnorm) && rnorm <= st->tol;
   const bool capped = st->maxiter >= 0 && k >= st->maxiter;
   const bool nan_stop = rnorm != rnorm && st->maxiter < 0;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (converged || capped || nan_stop) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lead) {
+      st->rr = s[0];
+      st->rz_new = s[1];
+      st->rnorm = rnorm;
       st->done = 1;
       st->iters = k;
     }
     return;
   }
-  const T beta = (T)(st->rz_new / st->rz_cur);
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  for (int kk = 0; kk < kRowsPerThread; ++kk) {
-    const int64_t i = b * kTile + kk * kB + threadIdx.x;
-    if (i >= n) break;
-    p[i] = z[i] + beta * p[i];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const T beta = (T)(s[1] / st->rz_cur);
+  ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
+  if (lead) {
+    st->rr = s[0];
+    st->rz_new = s[1];
+    st->rnorm = rnorm;
     st->k_next = k + 1;
-    st->rz_next = st->rz_new;
+    st->rz_next = s[1];
     st->beta = (double)beta;
   }
 }
@@ -217,82 +157,109 @@ __global__ __launch_bounds__(kB) void k_pcg_p(int64_t n, const T *__restrict__ z
 // ---------------------------------------------------------------------------
 // GMRES Arnoldi cycle
 // ---------------------------------------------------------------------------
-// ||b|| (Init, gmres.py:36-37)
+// Partials of ||b||^2 (Init, gmres.py:36-37); k_gm_bnorm_fin finishes.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_bnorm(int64_t n, const T *__restrict__ bvec,
-                                                GmresState *st, double *part) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kRB) void k_gm_bnorm(int64_t n, const T *__restrict__ bvec,
+                                                 double *__restrict__ part) {
   double acc[1] = {0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
-    acc[0] += (double)(bvec[i] * bvec[i]);
-  }
-  double tot[1];
-  if (grid_sum_last<1>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0)
-    st->bnorm = cr_sqrt(tot[0]);
+  ESR_GRID_ROWS(i, n) acc[0] += (double)(bvec[i] * bvec[i]);
+  store_partials<1>(acc, part);
 }
 
-// UpdateRnorm (gmres.py:22-25): r = b - A x; rnorm = ||r||. Clears the
-// cycle's breakdown flag.
+__global__ __launch_bounds__(kRB) void k_gm_bnorm_fin(int64_t g, const double *__restrict__ part,
+                                                     GmresState *st) {
+  double tot[1];
+  sum_partials<1>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) st->bnorm = cr_sqrt(tot[0]);
+}
+
+// UpdateRnorm (gmres.py:22-25): r = b - A x; partials of r.r;
+// k_gm_residual_fin finishes ||r|| and clears the cycle's breakdown flag.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_residual(
+__global__ __launch_bounds__(kRB) void k_gm_residual(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ bvec,
-    const T *__restrict__ x, T *__restrict__ r, GmresState *st, double *part) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const T *__restrict__ x, T *__restrict__ r, double *__restrict__ part) {
   double acc[1] = {0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
-    const T ri = bvec[i] - spmv_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
+  ESR_GRID_ROWS(i, n) {
+    const T ri = bvec[i] - linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
     r[i] = ri;
     acc[0] += (double)(ri * ri);
   }
+  store_partials<1>(acc, part);
+}
+
+__global__ __launch_bounds__(kRB) void k_gm_residual_fin(int64_t g,
+                                                        const double *__restrict__ part,
+                                                        GmresState *st) {
   double tot[1];
-  if (grid_sum_last<1>(acc, part, &st->counter[1], gridDim.x, b, tot) && threadIdx.x == 0) {
+  sum_partials<1>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) {
     st->rnorm = cr_sqrt(tot[0]);
     st->broken = 0;
     st->jend = -1;
   }
 }
 
-// InitV/UpdateV + InitW (gmres.py:48, 64, 120): V_j = src / s (s = rnorm for
-// j = 0, H[j, j-1] otherwise) and w = A(M(V_j)), M = identity or r / diag.
+// InitV/UpdateV + InitW (gmres.py:48, 64, 120): V_j = src / s and
+// w = A(M(V_j)), M = identity or r / diag. For j = 0, s = ||r||. For j > 0,
+// s = H[j, j-1] = ||w|| from the previous norm kernel's partials (stored in
+// H, gmres.py:90, 210-213); below 1e-15 the cycle breaks down at j - 1. The
+// breakdown flag is read and (by workgroup 0) written in this kernel, so both
+// accesses are atomic; every workgroup derives the same decision either way.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_spmv(
+__global__ __launch_bounds__(kRB) void k_gm_spmv(
     int64_t n, int j, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ diag,
-    const T *__restrict__ src, T *__restrict__ Vj, T *__restrict__ w, GmresState *st) {
-  if (st->broken) return;
-  const T s = (T)(j == 0 ? st->rnorm : st->H[j * kGmresMaxRestart + (j - 1)]);
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const T *__restrict__ src, T *__restrict__ Vj, T *__restrict__ w, GmresState *st,
+    const double *__restrict__ part_in) {
+  if (__hip_atomic_load(&st->broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  T s;
+  if (j == 0) {
+    s = (T)st->rnorm;
+  } else {
+    double tot[1];
+    sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
+    const double hn = (double)(T)cr_sqrt(tot[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->H[j * kGmresMaxRestart + (j - 1)] = hn;
+      st->jend = j - 1;
+      if (hn < 1e-15)
+        __hip_atomic_store(&st->broken, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (hn < 1e-15) return;
+    s = (T)hn;
+  }
   auto mv = [&](int64_t c) {
     const T v = src[c] / s;
     return diag ? v / diag[c] : v;
   };
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
+  ESR_GRID_ROWS(i, n) {
     Vj[i] = src[i] / s;
-    w[i] = spmv_row<T>(i, rowptr, col, Ae, Av, mv);
+    w[i] = linsys_row<T>(i, rowptr, col, Ae, Av, mv);
   }
 }
 
 // Modified Gram-Schmidt step i of Arnoldi step j (gmres.py:79, 105):
-// apply the pending w -= H[i-1, j] V_{i-1}, then H[i, j] = V_i . w.
+// H[i-1, j] from the previous projection's partials, w -= H[i-1, j] V_{i-1},
+// then partials of V_i . w.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_mgs(int64_t n, int i, int j,
-                                              const T *__restrict__ Vprev,
-                                              const T *__restrict__ Vi, T *__restrict__ w,
-                                              GmresState *st, double *part) {
+__global__ __launch_bounds__(kRB) void k_gm_mgs(int64_t n, int i, int j,
+                                               const T *__restrict__ Vprev,
+                                               const T *__restrict__ Vi, T *__restrict__ w,
+                                               GmresState *st, const double *__restrict__ part_in,
+                                               double *__restrict__ part_out) {
   if (st->broken) return;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const T hprev = i > 0 ? (T)st->H[(i - 1) * kGmresMaxRestart + j] : T(0);
+  T hprev = T(0);
+  if (i > 0) {
+    double tot[1];
+    sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
+    const double h = (double)(T)tot[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * kGmresMaxRestart + j] = h;
+    hprev = (T)h;
+  }
   double acc[1] = {0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t r = b * kTile + k * kB + threadIdx.x;
-    if (r >= n) break;
+  ESR_GRID_ROWS(r, n) {
     T wr = w[r];
     if (i > 0) {
       wr = wr - hprev * Vprev[r];
@@ -300,47 +267,53 @@ __global__ __launch_bounds__(kB) void k_gm_mgs(int64_t n, int i, int j,
     }
     acc[0] += (double)(Vi[r] * wr);
   }
-  double tot[1];
-  if (grid_sum_last<1>(acc, part, &st->counter[2], gridDim.x, b, tot) && threadIdx.x == 0)
-    st->H[i * kGmresMaxRestart + j] = (double)(T)tot[0];
+  store_partials<1>(acc, part_out);
 }
 
-// Last MGS update and NormW (gmres.py:90, 210-213): w -= H[j, j] V_j;
-// H[j+1, j] = ||w||; breakdown when it is below 1e-15.
+// Last MGS update (gmres.py:90): H[j, j] from the projection's partials,
+// w -= H[j, j] V_j, then partials of w.w (NormW, gmres.py:210-213).
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_norm(int64_t n, int j, const T *__restrict__ Vj,
-                                               T *__restrict__ w, GmresState *st,
-                                               double *part) {
+__global__ __launch_bounds__(kRB) void k_gm_norm(int64_t n, int j, const T *__restrict__ Vj,
+                                                T *__restrict__ w, GmresState *st,
+                                                const double *__restrict__ part_in,
+                                                double *__restrict__ part_out) {
   if (st->broken) return;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const T h = (T)st->H[j * kGmresMaxRestart + j];
+  double tot[1];
+  sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
+  const double hd = (double)(T)tot[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->H[j * kGmresMaxRestart + j] = hd;
+  const T h = (T)hd;
   double acc[1] = {0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t r = b * kTile + k * kB + threadIdx.x;
-    if (r >= n) break;
+  ESR_GRID_ROWS(r, n) {
     const T wr = w[r] - h * Vj[r];
     w[r] = wr;
     acc[0] += (double)(wr * wr);
   }
+  store_partials<1>(acc, part_out);
+}
+
+// End of a cycle of m steps: H[m, m-1] = ||w|| from the last norm's partials.
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_gm_fin(int64_t g, int m, GmresState *st,
+                                               const double *__restrict__ part_in) {
+  if (st->broken) return;
   double tot[1];
-  if (grid_sum_last<1>(acc, part, &st->counter[3], gridDim.x, b, tot) && threadIdx.x == 0) {
+  sum_partials<1>(part_in, g, 0, g, tot);
+  if (threadIdx.x == 0) {
     const double hn = (double)(T)cr_sqrt(tot[0]);
-    st->H[(j + 1) * kGmresMaxRestart + j] = hn;
-    st->jend = j;
+    st->H[m * kGmresMaxRestart + (m - 1)] = hn;
+    st->jend = m - 1;
     if (hn < 1e-15) st->broken = 1;
   }
 }
 
 // UpdateX (gmres.py:133-136): x += M(sum_{i<jc} V_i y_i).
 template <typename T>
-__global__ __launch_bounds__(kB) void k_gm_update_x(int64_t n, int jc, const T *__restrict__ V,
-                                                   const T *__restrict__ y,
-                                                   const T *__restrict__ diag,
-                                                   T *__restrict__ x) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t r = b * kTile + k * kB + threadIdx.x;
-    if (r >= n) break;
+__global__ __launch_bounds__(kRB) void k_gm_update_x(int64_t n, int jc, const T *__restrict__ V,
+                                                    const T *__restrict__ y,
+                                                    const T *__restrict__ diag,
+                                                    T *__restrict__ x) {
+  ESR_GRID_ROWS(r, n) {
     T dx = T(0);
     for (int i = 0; i < jc; ++i) dx = dx + V[(int64_t)i * n + r] * y[i];
     x[r] = x[r] + (diag ? dx / diag[r] : dx);
@@ -353,47 +326,44 @@ __global__ __launch_bounds__(kB) void k_gm_update_x(int64_t n, int jc, const T *
 struct IrState {
   double rnorm;
   double bnorm;
-  uint32_t counter[2];
 };
 
-// r = b - A x in float64; writes r rounded to float32 and ||r|| (float64);
-// with first != 0 also ||b||.
-__global__ __launch_bounds__(kB) void k_ir_residual(
+// r = b - A x in float64; writes r rounded to float32 and partials of r.r
+// and b.b; k_ir_fin finishes ||r|| (and, with first != 0, ||b||).
+__global__ __launch_bounds__(kRB) void k_ir_residual(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const double *__restrict__ Ae, const double *__restrict__ Av,
     const double *__restrict__ bvec, const double *__restrict__ x, float *__restrict__ r32,
-    IrState *st, double *part, int first) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
+  ESR_GRID_ROWS(i, n) {
     const double ri =
-        bvec[i] - spmv_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
+        bvec[i] - linsys_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
     r32[i] = (float)ri;
     acc[0] += ri * ri;
     acc[1] += bvec[i] * bvec[i];
   }
+  store_partials<2>(acc, part);
+}
+
+__global__ __launch_bounds__(kRB) void k_ir_fin(int64_t g, const double *__restrict__ part,
+                                               IrState *st, int first) {
   double tot[2];
-  if (grid_sum_last<2>(acc, part, &st->counter[0], gridDim.x, b, tot) && threadIdx.x == 0) {
+  sum_partials<2>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) {
     st->rnorm = cr_sqrt(tot[0]);
     if (first) st->bnorm = cr_sqrt(tot[1]);
   }
 }
 
 // x += (double) d; d = 0 (the next inner solve starts from zero).
-__global__ __launch_bounds__(kB) void k_ir_update(int64_t n, double *__restrict__ x,
-                                                 float *__restrict__ d) {
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  for (int k = 0; k < kRowsPerThread; ++k) {
-    const int64_t i = b * kTile + k * kB + threadIdx.x;
-    if (i >= n) break;
+__global__ __launch_bounds__(kRB) void k_ir_update(int64_t n, double *__restrict__ x,
+                                                  float *__restrict__ d) {
+  ESR_GRID_ROWS(i, n) {
     x[i] = x[i] + (double)d[i];
     d[i] = 0.0f;
   }
 }
-
-inline unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n, kTile); }
 
 } // namespace
 } // namespace esr
@@ -414,10 +384,13 @@ using namespace esr;
     }                                                                           \
   } while (0)
 
+// Workspace of the PCG kernels: partials of Init (2 G), Step part 1 (G) and
+// Step part 2 (2 G), G = red_grid(n) workgroups; part 1 and part 2 live in
+// different regions so a kernel never overwrites what it is still summing.
 ESR_API int esr_pcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_pcg_workspace: bad args");
   *state_bytes = (int64_t)sizeof(PcgState);
-  *ws_bytes = 2 * (int64_t)grid_of(n) * (int64_t)sizeof(double);
+  *ws_bytes = 3 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -433,13 +406,15 @@ ESR_API int esr_pcg_init(int dtype, int64_t n, const int32_t *rowptr, const int3
   h.rtol = rtol;
   h.atol = atol;
   h.maxiter = maxiter;
-  // parameters go in with a plain copy; the kernel fills the rest
+  // parameters go in with a plain copy; the kernels fill the rest
   ESR_HIP(hipMemcpyAsync(state, &h, sizeof(h), hipMemcpyHostToDevice, s));
   ESR_HIP(hipStreamSynchronize(s));
-  ESR_FLOAT_DISPATCH(dtype, (k_pcg_init<T><<<grid_of(n), kB, 0, s>>>(
+  const unsigned g = red_grid(n);
+  ESR_FLOAT_DISPATCH(dtype, (k_pcg_init<T><<<g, kRB, 0, s>>>(
                                 n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)diag,
                                 (const T *)b, (const T *)x, (T *)r, (T *)z, (T *)p,
-                                (PcgState *)state, (double *)ws)));
+                                (double *)ws)));
+  k_pcg_init_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (PcgState *)state);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
@@ -454,15 +429,16 @@ ESR_API int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr, const int
                   state && ws,
               "esr_pcg_steps: null argument");
   hipStream_t s = as_stream(stream);
-  const unsigned g = grid_of(n);
+  const unsigned g = red_grid(n);
+  double *part1 = (double *)ws, *part2 = part1 + g;
   for (int64_t it = 0; it < count; ++it) {
     ESR_FLOAT_DISPATCH(dtype, {
-      k_pcg_spmv<T><<<g, kB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
-                                     (const T *)p, (T *)Ap, (PcgState *)state, (double *)ws);
-      k_pcg_xr<T><<<g, kB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
-                                   (const T *)p, (const T *)Ap, (PcgState *)state,
-                                   (double *)ws);
-      k_pcg_p<T><<<g, kB, 0, s>>>(n, (const T *)z, (T *)p, (PcgState *)state);
+      k_pcg_spmv<T><<<g, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
+                                      (const T *)p, (T *)Ap, (PcgState *)state, part1);
+      k_pcg_xr<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
+                                    (const T *)p, (const T *)Ap, (PcgState *)state, part1,
+                                    part2);
+      k_pcg_p<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (PcgState *)state, part2);
     });
     ESR_LAUNCHED();
   }
@@ -488,13 +464,15 @@ ESR_API int esr_pcg_read(const void *state, double *scalars, int64_t *ints, void
   return ESR_OK;
 }
 
+// Workspace of the GMRES kernels: two regions of G partials used
+// alternately by consecutive reductions.
 ESR_API int esr_gmres_workspace(int64_t n, int64_t restart, int64_t *state_bytes,
                                 int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && restart >= 1 && state_bytes && ws_bytes, "esr_gmres_workspace: bad");
   ESR_REQUIRE(restart <= kGmresMaxRestart, "esr_gmres_workspace: restart %lld > %d",
               (long long)restart, kGmresMaxRestart);
   *state_bytes = (int64_t)sizeof(GmresState);
-  *ws_bytes = (int64_t)grid_of(n) * (int64_t)sizeof(double);
+  *ws_bytes = 2 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -506,13 +484,15 @@ ESR_API int esr_gmres_begin(int dtype, int64_t n, const void *b, void *state, vo
   GmresState h{};
   ESR_HIP(hipMemcpyAsync(state, &h, sizeof(h), hipMemcpyHostToDevice, s));
   ESR_HIP(hipStreamSynchronize(s));
-  ESR_FLOAT_DISPATCH(dtype, (k_gm_bnorm<T><<<grid_of(n), kB, 0, s>>>(
-                                n, (const T *)b, (GmresState *)state, (double *)ws)));
+  const unsigned g = red_grid(n);
+  ESR_FLOAT_DISPATCH(dtype, (k_gm_bnorm<T><<<g, kRB, 0, s>>>(n, (const T *)b, (double *)ws)));
+  k_gm_bnorm_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (GmresState *)state);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
 }
 
+// r = b - A x and ||r|| (a new cycle starts: breakdown flag cleared).
 ESR_API int esr_gmres_residual(int dtype, int64_t n, const int32_t *rowptr, const int32_t *col,
                                const void *Ae, const void *Av, const void *b, const void *x,
                                void *r, void *state, void *ws, void *stream) {
@@ -520,16 +500,20 @@ ESR_API int esr_gmres_residual(int dtype, int64_t n, const int32_t *rowptr, cons
   ESR_REQUIRE(n >= 1 && rowptr && col && Ae && Av && b && x && r && state && ws,
               "esr_gmres_residual: null argument");
   hipStream_t s = as_stream(stream);
-  ESR_FLOAT_DISPATCH(dtype, (k_gm_residual<T><<<grid_of(n), kB, 0, s>>>(
+  const unsigned g = red_grid(n);
+  ESR_FLOAT_DISPATCH(dtype, (k_gm_residual<T><<<g, kRB, 0, s>>>(
                                 n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)b,
-                                (const T *)x, (T *)r, (GmresState *)state, (double *)ws)));
+                                (const T *)x, (T *)r, (double *)ws)));
+  k_gm_residual_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (GmresState *)state);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
 }
 
-// One Arnoldi cycle: V (restart x n, one basis vector per row), w0/w1 two
-// n-vectors used alternately; every kernel is a no-op after a breakdown.
+// One Arnoldi cycle after esr_gmres_residual: V (restart x n, one basis
+// vector per row), w0/w1 two n-vectors used alternately; every kernel is a
+// no-op after a breakdown. The host reads ||r||, H, jend and the breakdown
+// flag afterwards (esr_gmres_read).
 ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t *rowptr,
                             const int32_t *col, const void *Ae, const void *Av,
                             const void *diag, const void *r, void *V, void *w0, void *w1,
@@ -540,21 +524,29 @@ ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t
   ESR_REQUIRE(rowptr && col && Ae && Av && r && V && w0 && w1 && state && ws,
               "esr_gmres_cycle: null argument");
   hipStream_t s = as_stream(stream);
-  const unsigned g = grid_of(n);
+  const unsigned g = red_grid(n);
+  double *reg[2] = {(double *)ws, (double *)ws + g};
   ESR_FLOAT_DISPATCH(dtype, {
     T *Vt = (T *)V;
     T *w[2] = {(T *)w0, (T *)w1};
     GmresState *st = (GmresState *)state;
+    int last = 0;  // region the latest reduction wrote (the residual: 0)
     for (int j = 0; j < (int)restart; ++j) {
       const T *src = j == 0 ? (const T *)r : w[(j - 1) & 1];
       T *wj = w[j & 1];
-      k_gm_spmv<T><<<g, kB, 0, s>>>(n, j, rowptr, col, (const T *)Ae, (const T *)Av,
-                                    (const T *)diag, src, Vt + (int64_t)j * n, wj, st);
-      for (int i = 0; i <= j; ++i)
-        k_gm_mgs<T><<<g, kB, 0, s>>>(n, i, j, i > 0 ? Vt + (int64_t)(i - 1) * n : Vt,
-                                     Vt + (int64_t)i * n, wj, st, (double *)ws);
-      k_gm_norm<T><<<g, kB, 0, s>>>(n, j, Vt + (int64_t)j * n, wj, st, (double *)ws);
+      k_gm_spmv<T><<<g, kRB, 0, s>>>(n, j, rowptr, col, (const T *)Ae, (const T *)Av,
+                                     (const T *)diag, src, Vt + (int64_t)j * n, wj, st,
+                                     reg[last]);
+      for (int i = 0; i <= j; ++i) {
+        k_gm_mgs<T><<<g, kRB, 0, s>>>(n, i, j, i > 0 ? Vt + (int64_t)(i - 1) * n : Vt,
+                                      Vt + (int64_t)i * n, wj, st, reg[last], reg[1 - last]);
+        last = 1 - last;
+      }
+      k_gm_norm<T><<<g, kRB, 0, s>>>(n, j, Vt + (int64_t)j * n, wj, st, reg[last],
+                                     reg[1 - last]);
+      last = 1 - last;
     }
+    k_gm_fin<T><<<1, kRB, 0, s>>>(g, (int)restart, st, reg[last]);
   });
   ESR_LAUNCHED();
   return ESR_OK;
@@ -567,7 +559,7 @@ ESR_API int esr_gmres_update_x(int dtype, int64_t n, int64_t jcount, const void 
   ESR_REQUIRE(n >= 1 && jcount >= 0 && jcount <= kGmresMaxRestart && V && y && x,
               "esr_gmres_update_x: bad argument");
   hipStream_t s = as_stream(stream);
-  ESR_FLOAT_DISPATCH(dtype, (k_gm_update_x<T><<<grid_of(n), kB, 0, s>>>(
+  ESR_FLOAT_DISPATCH(dtype, (k_gm_update_x<T><<<red_grid(n), kRB, 0, s>>>(
                                 n, (int)jcount, (const T *)V, (const T *)y,
                                 (const T *)diag, (T *)x)));
   ESR_LAUNCHED();
@@ -598,7 +590,7 @@ ESR_API int esr_gmres_read(const void *state, int64_t restart, double *scalars, 
 ESR_API int esr_ir_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_ir_workspace: bad args");
   *state_bytes = (int64_t)sizeof(IrState);
-  *ws_bytes = 2 * (int64_t)grid_of(n) * (int64_t)sizeof(double);
+  *ws_bytes = 2 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -609,13 +601,9 @@ ESR_API int esr_ir_residual(int64_t n, const int32_t *rowptr, const int32_t *col
   ESR_REQUIRE(n >= 1 && rowptr && col && Ae && Av && b && x && r32 && state && ws,
               "esr_ir_residual: null argument");
   hipStream_t s = as_stream(stream);
-  if (first) {
-    IrState h{};
-    ESR_HIP(hipMemcpyAsync(state, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    ESR_HIP(hipStreamSynchronize(s));
-  }
-  k_ir_residual<<<grid_of(n), kB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r32, (IrState *)state,
-                                          (double *)ws, first);
+  const unsigned g = red_grid(n);
+  k_ir_residual<<<g, kRB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r32, (double *)ws);
+  k_ir_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (IrState *)state, first);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
@@ -624,7 +612,7 @@ ESR_API int esr_ir_residual(int64_t n, const int32_t *rowptr, const int32_t *col
 ESR_API int esr_ir_update(int64_t n, double *x, float *d, void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(n >= 1 && x && d, "esr_ir_update: null argument");
-  k_ir_update<<<grid_of(n), kB, 0, as_stream(stream)>>>(n, x, d);
+  k_ir_update<<<red_grid(n), kRB, 0, as_stream(stream)>>>(n, x, d);
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END

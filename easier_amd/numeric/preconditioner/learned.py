@@ -43,6 +43,39 @@ def param_shapes(L):
     return shapes + [('w_out', (H,)), ('b_out', ())]
 
 
+def init_params(L=3, seed=0) -> Dict[str, torch.Tensor]:
+    """Initial float32 parameters for training: weights ~ N(0, 1/fan_in)
+    (Wu halved, so each residual layer starts as a small update), zero
+    biases and a zero decoder, so the untrained network is exactly Jacobi."""
+    rng = np.random.default_rng(seed)
+
+    def w(rows, cols, scale=1.0):
+        return torch.from_numpy((rng.standard_normal((rows, cols)) * scale /
+                                 np.sqrt(rows)).astype(np.float32))
+    p = {'W_in': w(3, H), 'b_in': torch.zeros(H)}
+    for l in range(L):
+        p[f'Wp{l}'], p[f'Wq{l}'], p[f'We{l}'] = w(H, H), w(H, H), w(3, H)
+        p[f'bm{l}'] = torch.zeros(H)
+        p[f'Wu{l}'], p[f'bu{l}'] = w(2 * H, H, 0.5), torch.zeros(H)
+    p['w_out'], p['b_out'] = torch.zeros(H), torch.zeros(())
+    return p
+
+
+def save(path, pre: 'LearnedPreconditioner', **meta):
+    """Parameters as an .npz of plain arrays (loadable without pickle)."""
+    np.savez(path, params=pre.params.detach().cpu().numpy(), L=np.int64(pre.L),
+             H=np.int64(H), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path, device=None) -> 'LearnedPreconditioner':
+    with np.load(path, allow_pickle=False) as f:
+        if int(f['H']) != H:
+            raise ValueError(f'{path}: hidden width {int(f["H"])}, this build has {H}')
+        L = int(f['L'])
+        flat = torch.from_numpy(f['params'].astype(np.float32))
+    return LearnedPreconditioner(L, unflatten(flat, L), device)
+
+
 def flatten(params: Dict[str, torch.Tensor], L) -> torch.Tensor:
     return torch.cat([params[k].reshape(-1).float() for k, _ in param_shapes(L)])
 
@@ -95,15 +128,17 @@ class MeshGraph:
 
 
 class LearnedPreconditioner:
+    _bindings = 0
+
     def __init__(self, L=3, params: Optional[Dict[str, torch.Tensor]] = None,
                  device=None):
+        self.binding = 0
         dev = device or torch.device('cuda', torch.cuda.current_device())
         npar, npk = ctypes.c_int64(), ctypes.c_int64()
         nat.call('esr_gnn_sizes', L, ctypes.byref(npar), ctypes.byref(npk))
         self.L, self.device = L, dev
         if params is None:
-            from_oracle = None
-            params = {k: torch.zeros(s) for k, s in param_shapes(L)}
+            params = init_params(L)
         flat = flatten(params, L)
         assert flat.numel() == npar.value, (flat.numel(), npar.value)
         self.params = flat.to(dev).contiguous()
@@ -124,6 +159,8 @@ class LearnedPreconditioner:
 
     # -- binding to a mesh --------------------------------------------------
     def bind(self, graph: MeshGraph, train=False):
+        LearnedPreconditioner._bindings += 1
+        self.binding = LearnedPreconditioner._bindings   # buffers below are new
         self.graph = graph
         n, L = graph.n, self.L
         nh = (L + 1) if train else 2
@@ -132,6 +169,9 @@ class LearnedPreconditioner:
         self.Q = torch.empty((nq, n, H), dtype=torch.bfloat16, device=self.device)
         self.a = torch.empty((L, n, H), dtype=torch.float32, device=self.device) if train else None
         self.rnorm = torch.empty(1, dtype=torch.float64, device=self.device)
+        probe = torch.empty((n, 1), dtype=torch.float64, device=self.device)
+        self.norm_ws = torch.empty(ops.aggregate_ws_bytes(probe), dtype=torch.uint8,
+                                   device=self.device)
         self.train = train
         if train:
             need = n * (8 * H + 3) + graph.nnz * H + 256 * 2 * H * H
@@ -139,10 +179,10 @@ class LearnedPreconditioner:
         return self
 
     def apply(self, r: torch.Tensor, z: torch.Tensor):
-        """z = M(r) (float64 device vectors of the bound mesh)."""
+        """z = M(r) (float64 device vectors of the bound mesh). Allocates
+        nothing, so it can be captured in a HIP graph."""
         g = self.graph
-        nrm = ops.aggregate('norm', r.reshape(-1, 1))
-        ops.copy_(self.rnorm, nrm.reshape(1))
+        ops.aggregate('norm', r.reshape(-1, 1), out=self.rnorm, ws=self.norm_ws)
         nat.call('esr_gnn_forward', self.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
                  g.efeat.data_ptr(), g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(),
                  self.rnorm.data_ptr(), self.params.data_ptr(), self.packed.data_ptr(),

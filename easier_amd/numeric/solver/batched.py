@@ -7,9 +7,10 @@ instances are concatenated into one CSR, processed by the same three kernels
 per iteration (SpMV + p.Ap; x/r/z + r.r, r.z; test + p), with per-instance
 device scalars and convergence masks: a converged instance's workgroups return
 at once. The iteration loop is one captured HIP graph replayed until the
-device counter of active instances reaches zero. The tile height (256*R rows)
-is picked by a small autotuner that times a few iterations of each candidate
-on the actual batch.
+device counter of active instances reaches zero. The number of workgroups per
+instance (about n_k / (256 R), R rows per thread) is picked by a small
+autotuner that times a few iterations of each candidate R on the actual
+batch.
 """
 import ctypes
 import time
@@ -34,8 +35,9 @@ _STATE_DTYPE = np.dtype([
     ('rr', 'f8'), ('rz_new', 'f8'), ('rnorm', 'f8'), ('bnorm', 'f8'),
     ('tol', 'f8'), ('rtol', 'f8'), ('atol', 'f8'), ('pad', 'f8'),
     ('k_cur', 'i8'), ('k_next', 'i8'), ('maxiter', 'i8'), ('done', 'i8'),
-    ('iters', 'i8'), ('pad2', 'i8'), ('counter', 'u4', (4,))])
+    ('iters', 'i8'), ('pad2', 'i8')])
 _CANDIDATES = (1, 2, 4, 8)
+_MAX_TILES = 256     # workgroups per instance (each sums its instance's partials)
 
 
 class BatchedCG:
@@ -89,23 +91,24 @@ class BatchedCG:
 
     # -- tiling -------------------------------------------------------------
     def _tiling(self, R):
+        """Instance k gets ceil(n_k / (256 R)) workgroups (at most
+        _MAX_TILES), which stride over its rows."""
         if R not in self.tiles:
             rows = 256 * R
             inst, r0, r1, first, nt = [], [], [], [], []
             for k in range(self.K):
                 o, n = int(self.offsets[k]), int(self.sizes[k])
                 first.append(len(inst))
-                cnt = (n + rows - 1) // rows
+                cnt = min(max((n + rows - 1) // rows, 1), _MAX_TILES)
                 nt.append(cnt)
-                for t in range(cnt):
-                    inst.append(k)
-                    r0.append(o + t * rows)
-                    r1.append(min(o + n, o + (t + 1) * rows))
+                inst.extend([k] * cnt)
+                r0.extend([o] * cnt)
+                r1.extend([o + n] * cnt)
             t32 = lambda a: torch.tensor(a, dtype=torch.int32, device=self.dev)
             ntiles = len(inst)
             self.tiles[R] = dict(n=ntiles, inst=t32(inst), row0=t32(r0), row1=t32(r1),
                                  first=t32(first), ntiles=t32(nt),
-                                 part=torch.empty(2 * ntiles, dtype=torch.float64,
+                                 part=torch.empty(3 * ntiles, dtype=torch.float64,
                                                   device=self.dev))
         return self.tiles[R]
 

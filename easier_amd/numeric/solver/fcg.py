@@ -30,11 +30,13 @@ from ..preconditioner.learned import LearnedPreconditioner, MeshGraph
 from .fused import LinsysCSR
 from .mixed import TrueResidual
 
-_c_i32, _c_i64, _c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+_c_i32, _c_i64, _c_vp, _c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
 nat.register_signatures({
     'esr_fcg_workspace': (_c_i32, [_c_i64, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
     'esr_fcg_op': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i32, _c_vp, _c_vp, _c_vp]),
-    'esr_fcg_read': (_c_i32, [_c_vp, ctypes.POINTER(ctypes.c_double), _c_vp]),
+    'esr_fcg_set': (_c_i32, [_c_vp, _c_dbl, _c_i64, _c_vp]),
+    'esr_fcg_read': (_c_i32, [_c_vp, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(_c_i64), _c_vp]),
 })
 
 SPMV, UPDATE, DOTS, DIRECTION, RESIDUAL, JACOBI = range(6)
@@ -42,9 +44,15 @@ SPMV, UPDATE, DOTS, DIRECTION, RESIDUAL, JACOBI = range(6)
 
 class FcgKernels:
     """Device vectors and kernels of one flexible-CG solve on a CSR Linsys
-    (rowptr, col, Ae, Av all on the device, float64)."""
+    (rowptr, col, Ae, Av all on the device, float64).
 
-    def __init__(self, rowptr, col, Ae, Av, n, device):
+    Steps with the learned preconditioner run in chunks of `chunk` steps
+    captured as one HIP graph (network forward included); the device stops
+    stepping once ||r|| <= tol, and the host checks progress once per chunk.
+    The Jacobi fallback hands the current iterate to the fused Jacobi-PCG
+    kernels (the plain solver), so the rest of the solve runs at its speed."""
+
+    def __init__(self, rowptr, col, Ae, Av, n, device, chunk=16):
         self.rowptr, self.col, self.Ae, self.Av, self.n = rowptr, col, Ae, Av, n
         mk = lambda: torch.zeros(n, dtype=torch.float64, device=device)
         self.r, self.ro, self.p, self.q, self.z = mk(), mk(), mk(), mk(), mk()
@@ -52,6 +60,11 @@ class FcgKernels:
         nat.call('esr_fcg_workspace', n, ctypes.byref(sb), ctypes.byref(wb))
         self.state = torch.zeros(sb.value, dtype=torch.uint8, device=device)
         self.ws = torch.empty(wb.value, dtype=torch.uint8, device=device)
+        nat.call('esr_pcg_workspace', n, ctypes.byref(sb), ctypes.byref(wb))
+        self.pcg_state = torch.zeros(sb.value, dtype=torch.uint8, device=device)
+        self.pcg_ws = torch.empty(wb.value, dtype=torch.uint8, device=device)
+        self.chunk = chunk
+        self.graphs = {}
 
     def op(self, code, b=None, x=None, a0=0):
         P = lambda t: None if t is None else t.data_ptr()
@@ -61,15 +74,59 @@ class FcgKernels:
 
     def read(self):
         sc = (ctypes.c_double * 9)()
-        nat.call('esr_fcg_read', self.state.data_ptr(), sc, nat.stream_ptr())
+        it = (ctypes.c_int64 * 3)()
+        nat.call('esr_fcg_read', self.state.data_ptr(), sc, it, nat.stream_ptr())
         keys = ('rz', 'pq', 'alpha', 'rr', 'zr', 'zro', 'beta', 'rr_true', 'bb')
-        return dict(zip(keys, sc))
+        out = dict(zip(keys, sc))
+        out.update(it=it[0], done=it[1], nbad=it[2])
+        return out
 
-    def precondition(self, M: Optional[LearnedPreconditioner], use_jacobi: bool):
-        if M is None or use_jacobi:
-            self.op(JACOBI)
+    def _step(self, M, x, restart):
+        """One flexible-CG step: z = M(r), beta, p, q = A p, alpha, x, r."""
+        M.apply(self.r, self.z)
+        self.op(DOTS, a0=1 if restart else 0)
+        if restart:
+            ops.copy_(self.p, self.z)
         else:
-            M.apply(self.r, self.z)
+            self.op(DIRECTION)
+        self.op(SPMV)
+        self.op(UPDATE, x=x)
+
+    def _chunk_graph(self, M, x):
+        # the graph holds the network's buffers of its current binding
+        key = (id(M), M.binding, x.data_ptr())
+        if key not in self.graphs:
+            from .fused import _Graph
+
+            def launch(sp):
+                for _ in range(self.chunk):
+                    self._step(M, x, False)
+            self.graphs[key] = (_Graph(launch), M)
+        return self.graphs[key][0]
+
+    def _jacobi_pcg(self, b, x, tol, maxiter):
+        """Fused Jacobi-PCG from the current x to ||r|| <= tol; returns
+        (steps, recursive residual)."""
+        from .fused import _Graph
+        P = lambda t: t.data_ptr()
+        csr = [P(self.rowptr), P(self.col), P(self.Ae), P(self.Av), P(self.Av)]
+        nat.call('esr_pcg_init', nat.F64, self.n, *csr, P(b), P(x), P(self.r), P(self.z),
+                 P(self.p), P(self.pcg_state), P(self.pcg_ws), 0.0, float(tol),
+                 -1 if maxiter is None else int(maxiter), nat.stream_ptr())
+        if 'pcg' not in self.graphs:
+            def launch(sp):
+                nat.call('esr_pcg_steps', nat.F64, self.n, *csr, P(x), P(self.r), P(self.z),
+                         P(self.p), P(self.q), P(self.pcg_state), P(self.pcg_ws), 32, sp)
+            self.graphs['pcg'] = (_Graph(launch), x.data_ptr())
+        g, xp = self.graphs['pcg']
+        assert xp == x.data_ptr()
+        sc = (ctypes.c_double * 6)()
+        it = (ctypes.c_int64 * 3)()
+        while True:
+            g.replay()
+            nat.call('esr_pcg_read', self.pcg_state.data_ptr(), sc, it, nat.stream_ptr())
+            if it[1]:
+                return int(it[0]), float(sc[0])
 
     def solve(self, b, x, M, rtol=1e-8, atol=None, maxiter=None, window=50,
               debug_iter=None):
@@ -78,48 +135,50 @@ class FcgKernels:
         bnorm = float(np.sqrt(s['bb']))
         tol = max(rtol * bnorm, atol) if atol else rtol * bnorm
         rnorm = float(np.sqrt(s['rr_true']))
-        use_jacobi = M is None
-        fallback_at = None
-        best, best_it = rnorm, 0
+        fallback_at, why = None, None
         it = 0
-        restart = True
-        while True:
-            if rnorm == rnorm and rnorm <= tol:
-                self.op(RESIDUAL, b=b, x=x)
+        if M is not None and not rnorm <= tol:
+            best, best_it = rnorm, 0
+            restart = True
+            while maxiter is None or it < maxiter:
+                left = -1 if maxiter is None else maxiter - it
+                nat.call('esr_fcg_set', self.state.data_ptr(), tol * tol, left,
+                         nat.stream_ptr())
+                if restart:
+                    self._step(M, x, True)
+                    restart = False
+                self._chunk_graph(M, x).replay()
                 s = self.read()
-                rnorm = float(np.sqrt(s['rr_true']))
+                it += s['it']
+                rnorm = float(np.sqrt(s['rr']))
+                if debug_iter:
+                    logger.info(f'FlexibleCG residual {rnorm} at step {it}')
+                if s['done'] and rnorm <= tol:
+                    self.op(RESIDUAL, b=b, x=x)          # judge on the true residual
+                    rnorm = float(np.sqrt(self.read()['rr_true']))
+                    if rnorm <= tol:
+                        break
+                    restart = True                       # drifted: restart from it
+                    continue
+                if rnorm < best * 0.999:
+                    best, best_it = rnorm, it
+                if s['nbad']:
+                    why = 'non-descent direction'
+                elif it - best_it > window:
+                    why = 'stagnation'
+                if why:
+                    fallback_at = it
+                    logger.info(f'FlexibleCG: falling back to Jacobi-PCG at step {it} ({why})')
+                    break
+        if M is None or why:
+            # the plain solver from the current iterate, judged on the true residual
+            while maxiter is None or it < maxiter:
+                k, _ = self._jacobi_pcg(b, x, tol, None if maxiter is None else maxiter - it)
+                it += max(k, 1)
+                self.op(RESIDUAL, b=b, x=x)
+                rnorm = float(np.sqrt(self.read()['rr_true']))
                 if rnorm <= tol:
                     break
-                restart = True          # recursive residual drifted: restart
-            if maxiter is not None and it >= maxiter:
-                break
-            self.precondition(M, use_jacobi)
-            self.op(DOTS, a0=1 if restart else 0)
-            s = self.read()
-            bad = not (s['zr'] > 0.0) or not np.isfinite(s['zr'])
-            stalled = it - best_it > window
-            if not use_jacobi and (bad or stalled):
-                use_jacobi = True
-                fallback_at = it
-                logger.info(f'FlexibleCG: falling back to Jacobi at step {it} '
-                            f'({"non-descent direction" if bad else "stagnation"})')
-                self.op(JACOBI)
-                self.op(DOTS, a0=1)
-                restart = True
-            if restart:
-                ops.copy_(self.p, self.z)
-                restart = False
-            else:
-                self.op(DIRECTION)
-            self.op(SPMV)
-            self.op(UPDATE, x=x)
-            it += 1
-            s = self.read()
-            rnorm = float(np.sqrt(s['rr']))
-            if rnorm < best * 0.999:
-                best, best_it = rnorm, it
-            if debug_iter and it % debug_iter == 0:
-                logger.info(f'FlexibleCG residual {rnorm} at step {it}')
         return {'residual': rnorm, 'iters': it, 'fallback_at': fallback_at,
                 'bnorm': bnorm}
 

@@ -5,14 +5,17 @@ Training pairs (r, e) come from the plain solver: Jacobi-PCG runs on seeded
 problems from x = 0 with a uniform random right-hand side; at a schedule of
 iterations k the residual r_k and the error e_k = x* - x_k (x* the converged
 solution, relative residual 1e-10) are recorded. The network is trained on
-the GPU with its own forward/backward kernels and an Adam step kernel to
-minimise mean ||M(r) - e||^2 / ||e||^2. Everything runs on device buffers in
-the mesh's natural order (no esr.compile renumbering is involved here).
+the GPU with its own forward/backward kernels and an Adam step kernel. The
+default loss is the A-norm error left after the best step along z = M(r),
+1 - (z.r)^2 / ((z.Az) ||e||_A^2), which is what one flexible-CG step gains;
+the plain error loss ||M(r) - e||^2 / ||e||^2 is the alternative. Training
+runs on device buffers in the mesh's natural order; evaluation goes through
+esr.compile like any user program.
 """
 import ctypes
 import math
 import time
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 import torch
@@ -92,9 +95,20 @@ class _Pcg:
                 return s
 
 
+def _dot(a, b):
+    return float(ops.aggregate('sum', ops.ewise('mul', a, b).reshape(-1, 1)).cpu())
+
+
+def spmv(prob: Problem, x: torch.Tensor) -> torch.Tensor:
+    """A x with the problem's operator (Linsys: off-diagonal CSR + diagonal)."""
+    g = ops.ewise('mul', ops.gather(x, prob.col), prob.Ae)
+    off = ops.csr_reduce(g, prob.rowptr, None, None, prob.n, 'sum')
+    return ops.ewise('add', off, ops.ewise('mul', x, prob.Av))
+
+
 def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=torch.float32):
-    """(r_k, e_k, ||e_k||^2) pairs from Jacobi-PCG runs with uniform random
-    right-hand sides (seeded)."""
+    """(r_k, e_k, ||e_k||^2, r_k.e_k) from Jacobi-PCG runs with uniform
+    random right-hand sides (seeded); r_k.e_k = ||e_k||_A^2 since A e_k = r_k."""
     pcg = _Pcg(prob)
     out = []
     gen = torch.Generator().manual_seed(seed)
@@ -117,15 +131,34 @@ def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=
         xstar = pcg.vec['x']
         for r, x in snaps:
             e = ops.ewise('sub', xstar, x.to(torch.float64))
-            en2 = float(ops.aggregate('sum', ops.ewise('mul', e, e).reshape(-1, 1)).cpu())
+            en2 = _dot(e, e)
             if en2 > 0:
-                out.append((r, e.to(store_dtype), en2))
+                out.append((r, e.to(store_dtype), en2, _dot(r, e)))
     return out
 
 
+def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
+    """(loss, dloss/dz) of the preconditioned vector z = M(r).
+
+    kind 'error':  ||z - e||^2 / ||e||^2.
+    kind 'energy': 1 - (z.r)^2 / ((z.Az) (r.e)) -- the fraction of the
+        A-norm error left after the best step along z (r.e = ||e||_A^2); its
+        gradient is -2 (z.r) / ((z.Az)(r.e)) (r - (z.r)/(z.Az) A z).
+        Scale-invariant in z, like the step flexible CG takes."""
+    if kind == 'error':
+        diff = ops.ewise('sub', z, e)
+        return _dot(diff, diff) / en2, ops.ewise('mul', diff, 2.0 / en2)
+    q = spmv(prob, z)
+    zr, zq = _dot(z, r), _dot(z, q)
+    loss = 1.0 - zr * zr / (zq * re)
+    t = ops.ewise('sub', r, ops.ewise('mul', q, zr / zq))
+    return loss, ops.ewise('mul', t, -2.0 * zr / (zq * re))
+
+
 def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every=200,
-          clip=1.0):
-    """data: list of (Problem, pairs). Returns the per-step losses."""
+          clip=1.0, loss='energy'):
+    """data: list of (Problem, pairs). Returns [(step, loss)] every
+    `log_every` steps (and the mean loss of each epoch as (step, mean))."""
     rng = np.random.default_rng(seed)
     losses = []
     step = 0
@@ -134,69 +167,154 @@ def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every
         rng.shuffle(order)
         order.sort(key=lambda t: t[0])           # one bind per problem per epoch
         bound = None
+        tot = 0.0
         for i, j in order:
             prob, pairs = data[i]
-            if bound is not i:
+            if bound != i:
                 pre.bind(prob.graph, train=True)
                 z = torch.empty(prob.n, dtype=torch.float64, device=prob.device)
                 bound = i
-            r, e, en2 = pairs[j]
+            r, e, en2, re = pairs[j]
             pre.apply(r, z)
-            diff = ops.ewise('sub', z, e)
-            dz = ops.ewise('mul', diff, 2.0 / en2)
+            l, dz = pair_loss(prob, r, e, en2, re, z, loss)
             pre.backward(r, dz)
             pre.adam_step(lr=lr, clip=clip)
             step += 1
+            tot += l
             if step % log_every == 0 or step == 1:
-                l = float(ops.aggregate('sum', ops.ewise('mul', diff, diff).reshape(-1, 1))
-                          .cpu()) / en2
                 losses.append((step, l))
+        losses.append((step, tot / max(len(order), 1)))
     return losses
 
 
-def evaluate(pre: LearnedPreconditioner, prob: Problem, rtol=1e-8, seed=123,
-             maxiter=None):
-    """Wall-clock of flexible CG with the learned preconditioner against
-    Jacobi-PCG (fused kernels) on the same right-hand side; both converge on
-    the true float64 residual."""
-    from ..numeric.solver.fcg import FcgKernels
-    gen = torch.Generator().manual_seed(seed)
-    b = torch.rand(prob.n, generator=gen, dtype=torch.float64) * 2 - 1
-    b[torch.from_numpy(prob.sys['boundary'])] = 0.0
-    bd = b.to(prob.device)
-    pcg = _Pcg(prob)
-    pcg.solve(bd, rtol)                      # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s = pcg.solve(bd, rtol)
-    torch.cuda.synchronize()
-    t_jac = time.perf_counter() - t0
-    x_jac = pcg.vec['x'].clone()
-    pre.bind(prob.graph)
-    fk = FcgKernels(prob.rowptr, prob.col, prob.Ae, prob.Av, prob.n, prob.device)
-    x = torch.zeros(prob.n, dtype=torch.float64, device=prob.device)
-    fk.solve(bd, x, pre, rtol, maxiter=maxiter)   # warm-up
-    ops.fill_(x, 0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    info = fk.solve(bd, x, pre, rtol, maxiter=maxiter)
-    torch.cuda.synchronize()
-    t_learn = time.perf_counter() - t0
-    err = float(ops.aggregate('norm', ops.ewise('sub', x, x_jac).reshape(-1, 1)).cpu()) / \
-        float(ops.aggregate('norm', x_jac.reshape(-1, 1)).cpu())
-    return dict(n=prob.n, jacobi_iters=s['iters'], jacobi_s=t_jac,
-                jacobi_residual=s['rnorm'], learned_iters=info['iters'], learned_s=t_learn,
-                learned_residual=info['residual'], fallback_at=info['fallback_at'],
-                speedup=t_jac / t_learn, rel_diff_x=err, bnorm=s['bnorm'])
-
-
-def jacobi_loss(data):
-    """The same loss for the untrained (Jacobi) preconditioner z = r/d."""
+def mean_loss(pre: Optional[LearnedPreconditioner], data, loss='energy'):
+    """Mean pair_loss over the pairs; pre None = the Jacobi preconditioner
+    z = r / d (the untrained network)."""
     tot, cnt = 0.0, 0
     for prob, pairs in data:
-        for r, e, en2 in pairs:
-            z = ops.ewise('div', r, prob.Av)
-            d = ops.ewise('sub', z, e)
-            tot += float(ops.aggregate('sum', ops.ewise('mul', d, d).reshape(-1, 1)).cpu()) / en2
+        if pre is not None:
+            pre.bind(prob.graph)
+        for r, e, en2, re in pairs:
+            z = ops.ewise('div', r, prob.Av) if pre is None else pre(r)
+            tot += pair_loss(prob, r, e, en2, re, z, loss)[0]
             cnt += 1
     return tot / max(cnt, 1)
+
+
+def evaluate_compiled(pre: LearnedPreconditioner, n, seed, rtol=1e-8, maxiter=None,
+                      reps=2):
+    """The drop-in comparison: the variable-coefficient problem of `seed` on
+    the n x n mesh, compiled with esr.compile(..., 'hip') twice -- Jacobi-PCG
+    (the fused CG solver) and flexible CG with the learned preconditioner --
+    both from x = 0 to `rtol` relative residual. Times are the best of `reps`
+    solves after one warm-up; the true float64 residual ||b - A x|| of both
+    final iterates is recomputed on the host."""
+    import scipy.sparse as sp
+    from .. import compile as esr_compile
+    from ..numeric.solver import CG, FlexibleCG
+    mesh, sys_, _ = wp.varcoef_system(n, seed)
+    logk = wp.gaussian_random_field(n, seed).reshape(-1)
+    nv = sys_['Av'].shape[0]
+    A = sp.csr_matrix((sys_['Ae'], (sys_['row'], sys_['col'])), shape=(nv, nv)) + \
+        sp.diags(sys_['Av'])
+    bnorm = float(np.linalg.norm(sys_['b']))
+    out = dict(n_grid=n, vertices=nv, seed=seed, rtol=rtol)
+
+    def timed(solve, x):
+        best, info = None, None
+        for rep in range(reps + 1):
+            ops.fill_(x.data, 0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            info = solve()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if rep > 0:
+                best = dt if best is None else min(best, dt)
+        return best, info
+
+    eqn = wp.PoissonP1(sys_)
+    cg = CG(eqn.A, eqn.b, eqn.x, eqn.M)
+    esr_compile([eqn, cg], 'hip')
+    t, info = timed(lambda: cg.solve(rtol=rtol, maxiter=maxiter), eqn.x)
+    xj = eqn.x.collect().numpy()
+    out['jacobi'] = dict(seconds=t, iters=info['iters'],
+                         true_rel_residual=float(np.linalg.norm(sys_['b'] - A @ xj)) / bnorm)
+    del eqn, cg
+    eqn = wp.PoissonP1(sys_)
+    fcg = FlexibleCG(eqn.A, eqn.b, eqn.x, pre,
+                     dict(points=mesh['points'], logk=logk, n_grid=n))
+    esr_compile([eqn, fcg], 'hip')
+    t, info = timed(lambda: fcg.solve(rtol=rtol, maxiter=maxiter), eqn.x)
+    xl = eqn.x.collect().numpy()
+    out['learned'] = dict(seconds=t, iters=info['iters'], fallback_at=info['fallback_at'],
+                          true_rel_residual=float(np.linalg.norm(sys_['b'] - A @ xl)) / bnorm)
+    out['speedup'] = out['jacobi']['seconds'] / out['learned']['seconds']
+    out['iteration_ratio'] = out['jacobi']['iters'] / max(out['learned']['iters'], 1)
+    out['rel_diff_x'] = float(np.linalg.norm(xl - xj) / np.linalg.norm(xj))
+    return out
+
+
+def _log(msg):
+    print(f'[learned {time.strftime("%H:%M:%S")}] {msg}', flush=True)
+
+
+def main(argv=None):
+    """python -m easier_amd.workloads.learned train|evaluate ..."""
+    import argparse
+    from ..numeric.preconditioner import learned as lp
+    ap = argparse.ArgumentParser(prog='python -m easier_amd.workloads.learned')
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    tr = sub.add_parser('train', help='generate (r, e) pairs with Jacobi-PCG on seeded '
+                        'problems and train the network on the GPU')
+    tr.add_argument('--n', type=int, default=512)
+    tr.add_argument('--problems', type=int, default=24)
+    tr.add_argument('--rhs', type=int, default=36)
+    tr.add_argument('--epochs', type=int, default=4)
+    tr.add_argument('--lr', type=float, default=1e-3)
+    tr.add_argument('--layers', type=int, default=3)
+    tr.add_argument('--seed0', type=int, default=1000)
+    tr.add_argument('--loss', choices=['energy', 'error'], default='energy')
+    tr.add_argument('--out', required=True)
+    ev = sub.add_parser('evaluate', help='learned FCG vs Jacobi-PCG on unseen seeds')
+    ev.add_argument('--weights', required=True)
+    ev.add_argument('--n', type=int, default=2048)
+    ev.add_argument('--seeds', type=int, nargs='+', default=[1])
+    ev.add_argument('--rtol', type=float, default=1e-8)
+    ev.add_argument('--maxiter', type=int, default=50000)
+    args = ap.parse_args(argv)
+    import json
+    if args.cmd == 'train':
+        data, t0 = [], time.perf_counter()
+        for k in range(args.problems):
+            prob = Problem(args.n, args.seed0 + k)
+            pairs = generate_pairs(prob, args.rhs, seed=args.seed0 + k)
+            data.append((prob, pairs))
+            _log(f'problem {k + 1}/{args.problems} (seed {args.seed0 + k}): {len(pairs)} '
+                 f'pairs, {time.perf_counter() - t0:.1f} s')
+        npairs = sum(len(p) for _, p in data)
+        base = mean_loss(None, data, args.loss)
+        _log(f'{npairs} pairs; Jacobi {args.loss} loss {base:.4f}')
+        pre = LearnedPreconditioner(args.layers, lp.init_params(args.layers, args.seed0))
+        for ep in range(args.epochs):
+            lr = args.lr * (0.5 ** ep)
+            losses = train(pre, data, epochs=1, lr=lr, seed=ep, log_every=1000,
+                           loss=args.loss)
+            _log(f'epoch {ep + 1}/{args.epochs} lr {lr:.2e}: mean {losses[-1][1]:.4f}; ' +
+                 ', '.join(f'{s}:{l:.4f}' for s, l in losses[-4:-1]) +
+                 f' ({time.perf_counter() - t0:.1f} s)')
+        final = mean_loss(pre, data, args.loss)
+        _log(f'trained {args.loss} loss {final:.4f} (Jacobi {base:.4f})')
+        lp.save(args.out, pre, n_train=args.n, pairs=npairs, jacobi_loss=base,
+                trained_loss=final, loss=args.loss,
+                seeds=np.arange(args.seed0, args.seed0 + args.problems))
+        _log(f'saved {args.out}')
+    else:
+        pre = lp.load(args.weights)
+        for s in args.seeds:
+            res = evaluate_compiled(pre, args.n, s, args.rtol, maxiter=args.maxiter)
+            print(json.dumps(res), flush=True)
+
+
+if __name__ == '__main__':
+    main()

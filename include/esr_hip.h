@@ -358,14 +358,19 @@ int esr_gnn_adam(int64_t np_, float *params, const float *grad, float *m,
  * op: 0 q = A p, p.q, alpha = rz / p.q; 1 r_old = r, r -= alpha q,
  * x += alpha p, ||r||^2; 2 z.r, z.r_old, beta (a0 != 0: restart, beta = 0),
  * rz = z.r; 3 p = z + beta p; 4 r = b - A x, ||r||^2, ||b||^2; 5 z = r / d.
- * esr_fcg_read: {rz, pq, alpha, rr, zr, zro, beta, rr_true, bb}.          */
+ * esr_fcg_set arms the step operations (0, 1, 2, 3, 5): they do nothing once
+ * a step has brought ||r||^2 to tol2 or maxit steps were taken (maxit < 0:
+ * no limit), so chunks of steps can be captured in a graph and replayed.
+ * esr_fcg_read: {rz, pq, alpha, rr, zr, zro, beta, rr_true, bb} and
+ * (ints nullable) {steps since esr_fcg_set, done, non-descent z.r <= 0}.   */
 int esr_fcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes);
 int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
                const double *Ae, const double *Av, const double *d,
                const double *b, double *x, double *r, double *ro, double *p,
                double *q, double *z, int a0, void *state, void *ws,
                void *stream);
-int esr_fcg_read(const void *state, double *scalars, void *stream);
+int esr_fcg_set(void *state, double tol2, int64_t maxit, void *stream);
+int esr_fcg_read(const void *state, double *scalars, int64_t *ints, void *stream);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,32 @@ def test_swe_step_fuses_into_four_kernels(dtype, kind):
     _rtc_ok(src)
 
 
+@pytest.mark.parametrize('opts', [
+    dict(stage_edges=False, nt_loads=False, nt_stores=False, block=256, xcd_remap=True),
+    dict(stage_edges=True, nt_loads=False, nt_stores=True, block=128, xcd_remap=False),
+    dict(stage_edges=True, nt_loads=True, nt_stores=False, block=512, xcd_remap=True)])
+def test_code_generation_options(opts):
+    mesh = wmesh.tri2(7)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81)
+    prog = compile_program([eqn], 'rcm', torch.device('cpu'))
+    saved = dict(fuser.OPTIONS)
+    try:
+        fuser.OPTIONS.update(opts)
+        p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    finally:
+        fuser.OPTIONS.update(saved)
+    assert all(k.block == opts['block'] for k in low.kernels)
+    assert all(k.staged == opts['stage_edges'] for k in low.kernels)
+    assert ('__syncthreads' in src) == opts['stage_edges']
+    assert ('__builtin_nontemporal_store' in src) == opts['nt_stores']
+    assert ('esr_remap(blockIdx.x' in src) == opts['xcd_remap']
+    # the staged edge range of one block fits its LDS arrays: 3 edges per row
+    for k in low.kernels:
+        assert k.lds_bytes <= fuser._LDS_BUDGET
+    _rtc_ok(src)
+
+
 def test_stores_wait_for_gathers_of_old_contents():
     class M(esr.Module):
         def __init__(self):
