@@ -256,14 +256,15 @@ def batched_config3(reps=1):
         ts = 0.0
         for s in systems[:8]:                # one at a time, on a sample
             single = BatchedCG([s])
-            single.R = R
+            single.R = R or 4
             dt, _ = _best_time(lambda: single.solve(rtol=1e-8, maxiter=20000), 1)
             ts += dt
         out[name] = dict(instances=len(systems), rows=int(bcg.offsets[-1]),
-                         tile_rows=256 * R, autotune_s=timings, solve_s=t,
-                         iters_max=max(its), iters_mean=float(np.mean(its)),
-                         all_converged=ok,
-                         sequential_first8_s=ts)
+                         solver='shared operator' if bcg.multi is not None else 'batched',
+                         tile_rows=None if R is None else 256 * R, autotune_ms=timings,
+                         solve_s=t, iters_max=max(its), iters_mean=float(np.mean(its)),
+                         all_converged=ok, sequential_first8_s=ts,
+                         sequential_all_est_s=ts * len(systems) / 8)
         log(f'config {name}: {t:.3f} s batched, iters max {max(its)}')
         del bcg
     return out

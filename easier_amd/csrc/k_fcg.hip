@@ -29,6 +29,7 @@ struct FcgState {
   int64_t it;     // ... steps taken since esr_fcg_set
   int64_t done;   // 1: the step operations are no-ops
   int64_t nbad;   // directions with z.r <= 0 or not finite since esr_fcg_set
+  int64_t use_jac; // this step's z was rejected: the direction uses r / d
 };
 
 namespace {
@@ -70,24 +71,39 @@ __global__ __launch_bounds__(kRB) void k_fcg_update(int64_t n, double *__restric
   store_partials<1>(acc, part);
 }
 
+// z.r, z.r_old and the same for the Jacobi vector j = r / d, which replaces
+// z for this step when z is not a descent direction (z.r <= 0 or not finite).
 __global__ __launch_bounds__(kRB) void k_fcg_dots(int64_t n, const double *__restrict__ z,
                                                  const double *__restrict__ r,
                                                  const double *__restrict__ ro,
+                                                 const double *__restrict__ d,
                                                  const FcgState *st, double *part) {
   if (st->done) return;
-  double acc[2] = {0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   ESR_GRID_ROWS(i, n) {
+    const double ji = r[i] / d[i];
     acc[0] += z[i] * r[i];
     acc[1] += z[i] * ro[i];
+    acc[2] += ji * r[i];
+    acc[3] += ji * ro[i];
   }
-  store_partials<2>(acc, part);
+  store_partials<4>(acc, part);
 }
 
+// p = s + beta p, or p = s on a restart; s = z, or r / d for a step whose z
+// was rejected.
 __global__ __launch_bounds__(kRB) void k_fcg_dir(int64_t n, const double *__restrict__ z,
-                                                double *__restrict__ p, const FcgState *st) {
+                                                const double *__restrict__ r,
+                                                const double *__restrict__ d,
+                                                double *__restrict__ p, const FcgState *st,
+                                                int restart) {
   if (st->done) return;
   const double beta = st->beta;
-  ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
+  const bool jac = st->use_jac != 0;
+  ESR_GRID_ROWS(i, n) {
+    const double si = jac ? r[i] / d[i] : z[i];
+    p[i] = restart ? si : si + beta * p[i];
+  }
 }
 
 // r = b - A x (float64); partials of ||r||^2 and ||b||^2
@@ -121,13 +137,18 @@ __global__ __launch_bounds__(kRB) void k_fcg_jacobi(int64_t n, const double *__r
 __global__ __launch_bounds__(kRB) void k_fcg_fin(int op, int64_t g, const double *part,
                                                 FcgState *st, int restart) {
   if (op != 4 && st->done) return;
-  double tot[2] = {0.0, 0.0};
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
   if (op == 0 || op == 1) {
-    double one[1];
-    sum_partials<1>(part, g, 0, g, one);
-    tot[0] = one[0];
+    double v[1];
+    sum_partials<1>(part, g, 0, g, v);
+    tot[0] = v[0];
+  } else if (op == 4) {
+    double v[2];
+    sum_partials<2>(part, g, 0, g, v);
+    tot[0] = v[0];
+    tot[1] = v[1];
   } else {
-    sum_partials<2>(part, g, 0, g, tot);
+    sum_partials<4>(part, g, 0, g, tot);
   }
   if (threadIdx.x != 0) return;
   switch (op) {
@@ -142,13 +163,17 @@ __global__ __launch_bounds__(kRB) void k_fcg_fin(int op, int64_t g, const double
     if (!(tot[0] > st->tol2) || (st->maxit >= 0 && it >= st->maxit)) st->done = 1;
     break;
   }
-  case 2:
-    st->zr = tot[0];
-    st->zro = tot[1];
-    st->beta = restart ? 0.0 : (tot[0] - tot[1]) / st->rz;
-    st->rz = tot[0];
-    if (!(tot[0] > 0.0) || !isfinite(tot[0])) st->nbad += 1;
+  case 2: {
+    const bool bad = !(tot[0] > 0.0) || !isfinite(tot[0]) || !isfinite(tot[1]);
+    const double zr = bad ? tot[2] : tot[0], zro = bad ? tot[3] : tot[1];
+    st->use_jac = bad ? 1 : 0;
+    if (bad) st->nbad += 1;
+    st->zr = zr;
+    st->zro = zro;
+    st->beta = restart ? 0.0 : (zr - zro) / st->rz;
+    st->rz = zr;
     break;
+  }
   case 4:
     st->rr_true = tot[0];
     st->rr = tot[0];
@@ -164,16 +189,17 @@ __global__ __launch_bounds__(kRB) void k_fcg_fin(int op, int64_t g, const double
 
 using namespace esr;
 
-// Partials: up to two values per workgroup, G = red_grid(n) workgroups.
+// Partials: up to four values per workgroup, G = red_grid(n) workgroups.
 ESR_API int esr_fcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_fcg_workspace: bad args");
   *state_bytes = (int64_t)sizeof(FcgState);
-  *ws_bytes = 2 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
+  *ws_bytes = 4 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
-// op codes: 0 spmv(p -> q), 1 update(x, r, ro, p, q), 2 dots(z, r, ro; a0 = restart),
-// 3 direction(z, p), 4 residual(b, x -> r), 5 jacobi(r -> z with d)
+// op codes: 0 spmv(p -> q), 1 update(x, r, ro, p, q), 2 dots(z, r, ro, d;
+// a0 = restart), 3 direction(z or r/d, p; a0 = restart), 4 residual(b, x -> r),
+// 5 jacobi(r -> z with d)
 ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
                        const double *Ae, const double *Av, const double *d, const double *b,
                        double *x, double *r, double *ro, double *p, double *q, double *z,
@@ -194,12 +220,12 @@ ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *
     k_fcg_update<<<g, kRB, 0, s>>>(n, x, r, ro, p, q, st, part);
     break;
   case 2:
-    ESR_REQUIRE(z && r && ro, "esr_fcg_op dots: null");
-    k_fcg_dots<<<g, kRB, 0, s>>>(n, z, r, ro, st, part);
+    ESR_REQUIRE(z && r && ro && d, "esr_fcg_op dots: null");
+    k_fcg_dots<<<g, kRB, 0, s>>>(n, z, r, ro, d, st, part);
     break;
   case 3:
-    ESR_REQUIRE(z && p, "esr_fcg_op direction: null");
-    k_fcg_dir<<<g, kRB, 0, s>>>(n, z, p, st);
+    ESR_REQUIRE(z && r && d && p, "esr_fcg_op direction: null");
+    k_fcg_dir<<<g, kRB, 0, s>>>(n, z, r, d, p, st, a0);
     break;
   case 4:
     ESR_REQUIRE(rowptr && col && Ae && Av && b && x && r, "esr_fcg_op residual: null");
@@ -245,6 +271,7 @@ __global__ void k_fcg_set(FcgState *st, double tol2, int64_t maxit) {
   st->it = 0;
   st->done = 0;
   st->nbad = 0;
+  st->use_jac = 0;
 }
 } // namespace
 

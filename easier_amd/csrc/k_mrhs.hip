@@ -1,0 +1,289 @@
+// Jacobi/identity-preconditioned CG for K right-hand sides of one Linsys
+// operator (easier/numeric/solver/cg.py:12-137 run on each right-hand side),
+// the operator read once per iteration for all of them.
+//
+// Vectors are stored right-hand-side-minor, v[row * K + k]. A row is served
+// by L = (K rounded up to a power of two) consecutive lanes, lane k holding
+// right-hand side k, so the row's column indices and stiffness values are
+// loaded once (the lanes read the same address) and every gathered
+// neighbour row is one contiguous K-value load. Each right-hand side keeps
+// its own scalars and convergence flag (BatchState, as the batched solver);
+// a converged right-hand side's lanes stop writing. Dot products: one
+// partial per (workgroup, right-hand side), summed in a fixed order by a
+// one-workgroup-per-right-hand-side kernel that also applies the scalar
+// step of cg.py (alpha; loop test, beta), so the row kernels read finished
+// scalars. Deterministic, no atomics on floats, no fences.
+#include "esr_common.h"
+#include "esr_krylov.h"
+
+namespace esr {
+
+struct MrhsState {  // per right-hand side (same layout as the batched solver's)
+  double rz_cur, rz_next, pAp, alpha, rr, rz_new, rnorm, bnorm, tol, rtol, atol, beta;
+  int64_t k_cur, k_next, maxiter, done, iters, pad2;
+};
+
+namespace {
+
+constexpr int kMaxK = 64;
+constexpr int kMaxGridM = 512;
+
+struct Lanes {
+  int L;        // lanes per row (power of two >= K)
+  int rows;     // rows per workgroup = kRB / L
+  int k;        // this thread's right-hand side
+  int rib;      // this thread's row within the workgroup's row group
+};
+
+__device__ __forceinline__ Lanes lanes_of(int K) {
+  int L = 1;
+  while (L < K) L <<= 1;
+  Lanes ln;
+  ln.L = L;
+  ln.rows = kRB / L;
+  ln.k = threadIdx.x % L;
+  ln.rib = threadIdx.x / L;
+  return ln;
+}
+
+#define ESR_MRHS_ROWS(i, n, ln)                                                       \
+  for (int64_t i = (int64_t)blockIdx.x * (ln).rows + (ln).rib; i < (n);              \
+       i += (int64_t)gridDim.x * (ln).rows)
+
+// Per-(workgroup, right-hand side) sums of NV per-thread values, fixed
+// order; part[(v * K + k) * G + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void store_rhs_partials(const double (&v)[NV], const Lanes &ln, int K,
+                                                   double *part) {
+  __shared__ double sh[NV][kRB];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  if (threadIdx.x < K) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int r = 0; r < ln.rows; ++r) s += sh[q][r * ln.L + threadIdx.x];
+      part[((int64_t)q * K + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+// Init (cg.py:25-29) per right-hand side: r = b - A x; z = M r; p = z.
+__global__ __launch_bounds__(kRB) void k_m_init(int64_t n, int K, const int32_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ col,
+                                               const double *__restrict__ Ae,
+                                               const double *__restrict__ Av,
+                                               const double *__restrict__ diag,
+                                               const double *__restrict__ B,
+                                               const double *__restrict__ X,
+                                               double *__restrict__ R, double *__restrict__ Z,
+                                               double *__restrict__ P, double *__restrict__ part) {
+  const Lanes ln = lanes_of(K);
+  const bool on = ln.k < K;
+  double acc[2] = {0.0, 0.0};
+  ESR_MRHS_ROWS(i, n, ln) {
+    if (!on) continue;
+    const double ax =
+        linsys_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return X[c * K + ln.k]; });
+    const double b = B[i * K + ln.k];
+    const double ri = b - ax;
+    const double zi = diag ? ri / diag[i] : ri;
+    R[i * K + ln.k] = ri;
+    Z[i * K + ln.k] = zi;
+    P[i * K + ln.k] = zi;
+    acc[0] += ri * zi;
+    acc[1] += b * b;
+  }
+  store_rhs_partials<2>(acc, ln, K, part);
+}
+
+// One workgroup per right-hand side: rz, bnorm, tol (cg.py:112-115).
+__global__ __launch_bounds__(kRB) void k_m_init_fin(int K, int64_t g, const double *__restrict__ part,
+                                                   MrhsState *st, int32_t *active) {
+  const int k = blockIdx.x;
+  double rz[1], bb[1];
+  sum_partials<1>(part, 0, (int64_t)k * g, g, rz);
+  sum_partials<1>(part, 0, ((int64_t)K + k) * g, g, bb);
+  if (threadIdx.x == 0) {
+    MrhsState *s = st + k;
+    s->rz_cur = s->rz_next = rz[0];
+    s->bnorm = cr_sqrt(bb[0]);
+    double tol = s->rtol * s->bnorm;
+    if (s->atol > 0.0 && s->atol > tol) tol = s->atol;
+    s->tol = tol;
+    s->k_cur = s->k_next = 0;
+    s->done = 0;
+    s->iters = 0;
+    if (k == 0) *active = K;
+  }
+}
+
+// Step part 1 (cg.py:44-47): AP = A P; partials of p.Ap.
+__global__ __launch_bounds__(kRB) void k_m_spmv(int64_t n, int K, const int32_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ col,
+                                               const double *__restrict__ Ae,
+                                               const double *__restrict__ Av,
+                                               const double *__restrict__ P,
+                                               double *__restrict__ AP, const MrhsState *st,
+                                               double *__restrict__ part) {
+  const Lanes ln = lanes_of(K);
+  const bool on = ln.k < K && !st[ln.k < K ? ln.k : 0].done;
+  double acc[1] = {0.0};
+  ESR_MRHS_ROWS(i, n, ln) {
+    if (!on) continue;
+    const double y =
+        linsys_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return P[c * K + ln.k]; });
+    AP[i * K + ln.k] = y;
+    acc[0] += P[i * K + ln.k] * y;
+  }
+  store_rhs_partials<1>(acc, ln, K, part);
+}
+
+// alpha = rz / pAp per right-hand side (one workgroup each); also commits
+// the counters of the previous Update.
+__global__ __launch_bounds__(kRB) void k_m_alpha(int K, int64_t g, const double *__restrict__ part,
+                                                MrhsState *st) {
+  const int k = blockIdx.x;
+  MrhsState *s = st + k;
+  if (s->done) return;
+  double tot[1];
+  sum_partials<1>(part, 0, (int64_t)k * g, g, tot);
+  if (threadIdx.x == 0) {
+    s->k_cur = s->k_next;
+    s->rz_cur = s->rz_next;
+    s->pAp = tot[0];
+    s->alpha = s->rz_cur / tot[0];
+  }
+}
+
+// Step part 2 (cg.py:47-52) + first half of Update: x, r, z; partials of
+// r.r and r.z.
+__global__ __launch_bounds__(kRB) void k_m_xr(int64_t n, int K, const double *__restrict__ diag,
+                                             double *__restrict__ X, double *__restrict__ R,
+                                             double *__restrict__ Z, const double *__restrict__ P,
+                                             const double *__restrict__ AP, const MrhsState *st,
+                                             double *__restrict__ part) {
+  const Lanes ln = lanes_of(K);
+  const MrhsState *s = st + (ln.k < K ? ln.k : 0);
+  const bool on = ln.k < K && !s->done;
+  const double alpha = s->alpha;
+  double acc[2] = {0.0, 0.0};
+  ESR_MRHS_ROWS(i, n, ln) {
+    if (!on) continue;
+    const int64_t o = i * K + ln.k;
+    const double ri = R[o] - alpha * AP[o];
+    X[o] = X[o] + alpha * P[o];
+    R[o] = ri;
+    const double zi = diag ? ri / diag[i] : ri;
+    Z[o] = zi;
+    acc[0] += ri * ri;
+    acc[1] += ri * zi;
+  }
+  store_rhs_partials<2>(acc, ln, K, part);
+}
+
+// Loop test (cg.py:126-129) and the scalars of Update per right-hand side.
+__global__ __launch_bounds__(kRB) void k_m_test(int K, int64_t g, const double *__restrict__ part,
+                                               MrhsState *st, int32_t *active) {
+  const int k = blockIdx.x;
+  MrhsState *s = st + k;
+  if (s->done) return;
+  double rr[1], rz[1];
+  sum_partials<1>(part, 0, (int64_t)k * g, g, rr);
+  sum_partials<1>(part, 0, ((int64_t)K + k) * g, g, rz);
+  if (threadIdx.x != 0) return;
+  const double rnorm = cr_sqrt(rr[0]);
+  const int64_t kk = s->k_cur;
+  s->rr = rr[0];
+  s->rz_new = rz[0];
+  s->rnorm = rnorm;
+  const bool stop = ((rnorm == rnorm) && rnorm <= s->tol) ||
+                    (s->maxiter >= 0 && kk >= s->maxiter) || (rnorm != rnorm && s->maxiter < 0);
+  if (stop) {
+    s->done = 1;
+    s->iters = kk;
+    atomicSub(active, 1);
+    return;
+  }
+  s->beta = rz[0] / s->rz_cur;
+  s->k_next = kk + 1;
+  s->rz_next = rz[0];
+}
+
+// Update (cg.py:67-71): p = z + beta p.
+__global__ __launch_bounds__(kRB) void k_m_p(int64_t n, int K, const double *__restrict__ Z,
+                                            double *__restrict__ P, const MrhsState *st) {
+  const Lanes ln = lanes_of(K);
+  const MrhsState *s = st + (ln.k < K ? ln.k : 0);
+  const bool on = ln.k < K && !s->done;
+  const double beta = s->beta;
+  ESR_MRHS_ROWS(i, n, ln) {
+    if (!on) continue;
+    const int64_t o = i * K + ln.k;
+    P[o] = Z[o] + beta * P[o];
+  }
+}
+
+inline unsigned mrhs_grid(int64_t n, int K) {
+  int L = 1;
+  while (L < K) L <<= 1;
+  const int64_t g = ceil_div(n, kRB / L);
+  return (unsigned)(g < kMaxGridM ? (g > 0 ? g : 1) : kMaxGridM);
+}
+
+} // namespace
+} // namespace esr
+
+using namespace esr;
+
+ESR_API int esr_mrhs_sizes(int64_t n, int64_t K, int64_t *state_bytes, int64_t *ws_bytes) {
+  ESR_REQUIRE(n >= 1 && K >= 1 && K <= kMaxK && state_bytes && ws_bytes,
+              "esr_mrhs_sizes: need n >= 1 and 1 <= K <= %d", kMaxK);
+  *state_bytes = K * (int64_t)sizeof(MrhsState);
+  *ws_bytes = 2 * K * (int64_t)mrhs_grid(n, (int)K) * (int64_t)sizeof(double);
+  return ESR_OK;
+}
+
+// B, X, R, Z, P, AP: (n, K) row-major float64; state: K MrhsState with rtol,
+// atol, maxiter set by the caller; active: one int32 (set to K).
+ESR_API int esr_mrhs_init(int64_t n, int64_t K, const int32_t *rowptr, const int32_t *col,
+                          const double *Ae, const double *Av, const double *diag,
+                          const double *B, const double *X, double *R, double *Z, double *P,
+                          void *state, double *ws, int32_t *active, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && K >= 1 && K <= kMaxK, "esr_mrhs_init: bad sizes");
+  ESR_REQUIRE(rowptr && col && Ae && Av && B && X && R && Z && P && state && ws && active,
+              "esr_mrhs_init: null argument");
+  hipStream_t s = as_stream(stream);
+  const unsigned g = mrhs_grid(n, (int)K);
+  k_m_init<<<g, kRB, 0, s>>>(n, (int)K, rowptr, col, Ae, Av, diag, B, X, R, Z, P, ws);
+  k_m_init_fin<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, (MrhsState *)state, active);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+// `count` iterations; each is a no-op for a converged right-hand side.
+ESR_API int esr_mrhs_steps(int64_t n, int64_t K, const int32_t *rowptr, const int32_t *col,
+                           const double *Ae, const double *Av, const double *diag, double *X,
+                           double *R, double *Z, double *P, double *AP, void *state, double *ws,
+                           int32_t *active, int64_t count, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && K >= 1 && K <= kMaxK && count >= 0, "esr_mrhs_steps: bad sizes");
+  ESR_REQUIRE(rowptr && col && Ae && Av && X && R && Z && P && AP && state && ws && active,
+              "esr_mrhs_steps: null argument");
+  hipStream_t s = as_stream(stream);
+  const unsigned g = mrhs_grid(n, (int)K);
+  MrhsState *st = (MrhsState *)state;
+  for (int64_t it = 0; it < count; ++it) {
+    k_m_spmv<<<g, kRB, 0, s>>>(n, (int)K, rowptr, col, Ae, Av, P, AP, st, ws);
+    k_m_alpha<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, st);
+    k_m_xr<<<g, kRB, 0, s>>>(n, (int)K, diag, X, R, Z, P, AP, st, ws);
+    k_m_test<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, st, active);
+    k_m_p<<<g, kRB, 0, s>>>(n, (int)K, Z, P, st);
+    ESR_LAUNCHED();
+  }
+  return ESR_OK;
+  ESR_GUARD_END
+}

@@ -154,6 +154,46 @@ __global__ __launch_bounds__(kRB) void k_pcg_p(int64_t n, const T *__restrict__ 
   }
 }
 
+// Re-arm after a reliable update (the caller replaced r by the true residual
+// of the current iterate): z = M r and partials of r.z. k_pcg_rearm_fin
+// takes the Update of cg.py:67-71 with this residual -- beta = r.z / rz of
+// the last step -- and installs the new exit tolerance; k_pcg_rearm_p sets
+// p = z + beta p, so the search directions stay conjugate.
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_pcg_rearm(int64_t n, const T *__restrict__ diag,
+                                                  const T *__restrict__ r, T *__restrict__ z,
+                                                  double *__restrict__ part) {
+  double acc[1] = {0.0};
+  ESR_GRID_ROWS(i, n) {
+    const T ri = r[i];
+    const T zi = diag ? ri / diag[i] : ri;
+    z[i] = zi;
+    acc[0] += (double)(ri * zi);
+  }
+  store_partials<1>(acc, part);
+}
+
+__global__ __launch_bounds__(kRB) void k_pcg_rearm_fin(int64_t g, const double *__restrict__ part,
+                                                      PcgState *st, double tol) {
+  double tot[1];
+  sum_partials<1>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) {
+    st->beta = tot[0] / st->rz_cur;
+    st->rz_new = tot[0];
+    st->rz_next = tot[0];
+    st->tol = tol;
+    st->k_next = st->k_cur + 1;
+    st->done = 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_pcg_rearm_p(int64_t n, const T *__restrict__ z,
+                                                    T *__restrict__ p, const PcgState *st) {
+  const T beta = (T)st->beta;
+  ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
+}
+
 // ---------------------------------------------------------------------------
 // GMRES Arnoldi cycle
 // ---------------------------------------------------------------------------
@@ -442,6 +482,25 @@ ESR_API int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr, const int
     });
     ESR_LAUNCHED();
   }
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+// Continue a stopped PCG after the caller replaced r by a freshly computed
+// residual (a reliable update): z = M r, beta = r.z / rz, p = z + beta p,
+// new absolute exit tolerance `tol`; the step counter continues.
+ESR_API int esr_pcg_rearm(int dtype, int64_t n, const void *diag, const void *r, void *z,
+                          void *p, void *state, void *ws, double tol, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && r && z && p && state && ws, "esr_pcg_rearm: null argument");
+  hipStream_t s = as_stream(stream);
+  const unsigned g = red_grid(n);
+  ESR_FLOAT_DISPATCH(dtype, {
+    k_pcg_rearm<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (const T *)r, (T *)z, (double *)ws);
+    k_pcg_rearm_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (PcgState *)state, tol);
+    k_pcg_rearm_p<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (const PcgState *)state);
+  });
+  ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
 }

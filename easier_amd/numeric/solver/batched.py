@@ -28,6 +28,10 @@ nat.register_signatures({
     'esr_bcg_state_bytes': (_c_i32, [ctypes.POINTER(_c_i64)]),
     'esr_bcg_init': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 17 + [_c_vp]),
     'esr_bcg_steps': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 18 + [_c_i64, _c_vp]),
+    'esr_mrhs_sizes': (_c_i32, [_c_i64, _c_i64, ctypes.POINTER(_c_i64),
+                                ctypes.POINTER(_c_i64)]),
+    'esr_mrhs_init': (_c_i32, [_c_i64, _c_i64] + [_c_vp] * 14),
+    'esr_mrhs_steps': (_c_i32, [_c_i64, _c_i64] + [_c_vp] * 13 + [_c_i64, _c_vp]),
 })
 
 _STATE_DTYPE = np.dtype([
@@ -40,15 +44,109 @@ _CANDIDATES = (1, 2, 4, 8)
 _MAX_TILES = 256     # workgroups per instance (each sums its instance's partials)
 
 
+def _same_operator(systems) -> bool:
+    a = systems[0]
+    for s in systems[1:]:
+        for k in ('row', 'col', 'Ae', 'Av'):
+            if s[k] is not a[k] and not (s[k].shape == a[k].shape and
+                                         np.array_equal(s[k], a[k])):
+                return False
+    return True
+
+
+class MultiRhsCG:
+    """Jacobi-PCG for K <= 64 right-hand sides of one operator, the operator
+    read once per iteration for all of them (csrc/k_mrhs.hip): vectors are
+    (n, K), each right-hand side keeps its own scalars, tolerance and
+    iteration count (cg.py:106-137 per right-hand side)."""
+
+    MAX_K = 64
+
+    def __init__(self, system: Dict[str, np.ndarray], B: np.ndarray, jacobi=True,
+                 device=None):
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        n = system['Av'].shape[0]
+        B = np.ascontiguousarray(B, np.float64).reshape(n, -1)
+        self.n, self.K = n, B.shape[1]
+        if not 1 <= self.K <= self.MAX_K:
+            raise ValueError(f'1 <= K <= {self.MAX_K} right-hand sides, got {self.K}')
+        if np.any(np.diff(system['row']) < 0):
+            raise ValueError('system rows must be sorted')
+        rowptr = np.concatenate([[0], np.cumsum(np.bincount(system['row'], minlength=n))])
+        t32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+        self.rowptr, self.col = t32(rowptr), t32(system['col'])
+        self.Ae, self.Av = f64(system['Ae']), f64(system['Av'])
+        self.diag = self.Av if jacobi else None
+        self.B = f64(B)
+        mk = lambda: torch.zeros((n, self.K), dtype=torch.float64, device=dev)
+        self.X, self.R, self.Z, self.P, self.AP = mk(), mk(), mk(), mk(), mk()
+        sb, wb = ctypes.c_int64(), ctypes.c_int64()
+        nat.call('esr_mrhs_sizes', n, self.K, ctypes.byref(sb), ctypes.byref(wb))
+        assert sb.value == self.K * _STATE_DTYPE.itemsize
+        self.state = torch.zeros(sb.value, dtype=torch.uint8, device=dev)
+        self.ws = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+        self.active = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.dev = dev
+        self.graphs = {}
+
+    def _op(self):
+        return [self.rowptr.data_ptr(), self.col.data_ptr(), self.Ae.data_ptr(),
+                self.Av.data_ptr(), None if self.diag is None else self.diag.data_ptr()]
+
+    def _steps(self, count, sp):
+        nat.call('esr_mrhs_steps', self.n, self.K, *self._op(), self.X.data_ptr(),
+                 self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(), self.AP.data_ptr(),
+                 self.state.data_ptr(), self.ws.data_ptr(), self.active.data_ptr(), count, sp)
+
+    def solve(self, rtol=1e-8, atol=None, maxiter=None, chunk=32):
+        st = np.zeros(self.K, _STATE_DTYPE)
+        st['rtol'] = rtol
+        st['atol'] = atol if atol else 0.0
+        st['maxiter'] = -1 if maxiter is None else maxiter
+        self.state.copy_(torch.from_numpy(st.view(np.uint8)).to(self.dev))
+        ops.fill_(self.X, 0)
+        nat.call('esr_mrhs_init', self.n, self.K, *self._op(), self.B.data_ptr(),
+                 self.X.data_ptr(), self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(),
+                 self.state.data_ptr(), self.ws.data_ptr(), self.active.data_ptr(),
+                 nat.stream_ptr())
+        if chunk not in self.graphs:
+            self.graphs[chunk] = _Graph(lambda sp: self._steps(chunk, sp))
+        g = self.graphs[chunk]
+        while True:
+            g.replay()
+            if int(self.active.item()) == 0:
+                break
+        st = self.state.cpu().numpy().view(_STATE_DTYPE)
+        return [{'residual': float(st['rnorm'][k]), 'iters': int(st['iters'][k])}
+                for k in range(self.K)]
+
+    def solutions(self):
+        X = self.X.cpu().numpy()
+        return [np.ascontiguousarray(X[:, k]) for k in range(self.K)]
+
+
 class BatchedCG:
     """systems: list of dicts with host arrays row, col (int64, CSR order:
     sorted by row), Ae, Av, b (float64), as workloads.poisson.p1_system
     returns. Solutions land in `x` (one float64 array per instance via
-    solutions())."""
+    solutions()). When every system has the same operator (right-hand sides
+    of one mesh, at most 64), the shared-operator solver MultiRhsCG runs
+    instead."""
 
     def __init__(self, systems: List[Dict[str, np.ndarray]], jacobi=True,
-                 device=None):
+                 device=None, share_operator=True):
         dev = device or torch.device('cuda', torch.cuda.current_device())
+        self.multi = None
+        if share_operator and 1 < len(systems) <= MultiRhsCG.MAX_K and \
+                _same_operator(systems):
+            B = np.stack([s['b'] for s in systems], 1)
+            self.multi = MultiRhsCG(systems[0], B, jacobi, dev)
+            self.K = len(systems)
+            self.sizes = np.full(self.K, systems[0]['Av'].shape[0], np.int64)
+            self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+            self.R = None
+            return
         sb = ctypes.c_int64()
         nat.call('esr_bcg_state_bytes', ctypes.byref(sb))
         assert sb.value == _STATE_DTYPE.itemsize, (sb.value, _STATE_DTYPE.itemsize)
@@ -142,7 +240,10 @@ class BatchedCG:
                  self.active.data_ptr(), count, sp)
 
     def autotune(self, iters=5):
-        """Pick the tile height with the fastest iterations on this batch."""
+        """Pick the tile height with the fastest iterations on this batch
+        (nothing to tune for the shared-operator solver)."""
+        if self.multi is not None:
+            return None, {}
         best = None
         timings = {}
         for R in _CANDIDATES:
@@ -166,6 +267,8 @@ class BatchedCG:
     def solve(self, rtol=1e-8, atol=None, maxiter=None, chunk=32):
         """Solve every instance from x = 0; returns a list of
         {'residual', 'iters'} (cg.py:137 per instance)."""
+        if self.multi is not None:
+            return self.multi.solve(rtol, atol, maxiter, chunk)
         R = self.R or 4
         ops.fill_(self.x, 0)
         self._init(R, rtol, atol, maxiter)
@@ -182,5 +285,7 @@ class BatchedCG:
                 for k in range(self.K)]
 
     def solutions(self):
+        if self.multi is not None:
+            return self.multi.solutions()
         x = self.x.cpu().numpy()
         return [x[self.offsets[k]:self.offsets[k + 1]] for k in range(self.K)]

@@ -10,10 +10,11 @@ a fixed preconditioner). Each step:
 
 Convergence is declared only on the true float64 residual ||b - A x||,
 recomputed with the plain operator when the recursive residual reaches the
-tolerance (a miss restarts from the true residual). The solver falls back to
-Jacobi-PCG for the rest of the solve when the network returns a non-descent
-direction (z.r <= 0 or not finite) or when the residual has not improved for
-`window` steps.
+tolerance (a miss restarts from the true residual). A network output that is
+not a descent direction (z.r <= 0 or not finite) is replaced by the Jacobi
+vector r / d for that step, on the device. The solver hands the rest of the
+solve to the fused Jacobi-PCG when the network was rejected for most steps
+of a chunk or when the residual has not improved for `window` steps.
 """
 import ctypes
 from typing import Any, Dict, Optional
@@ -82,13 +83,12 @@ class FcgKernels:
         return out
 
     def _step(self, M, x, restart):
-        """One flexible-CG step: z = M(r), beta, p, q = A p, alpha, x, r."""
+        """One flexible-CG step: z = M(r), beta, p, q = A p, alpha, x, r. A
+        z that is not a descent direction is replaced by r / d (Jacobi) for
+        this step, on the device."""
         M.apply(self.r, self.z)
         self.op(DOTS, a0=1 if restart else 0)
-        if restart:
-            ops.copy_(self.p, self.z)
-        else:
-            self.op(DIRECTION)
+        self.op(DIRECTION, a0=1 if restart else 0)
         self.op(SPMV)
         self.op(UPDATE, x=x)
 
@@ -128,14 +128,14 @@ class FcgKernels:
             if it[1]:
                 return int(it[0]), float(sc[0])
 
-    def solve(self, b, x, M, rtol=1e-8, atol=None, maxiter=None, window=50,
+    def solve(self, b, x, M, rtol=1e-8, atol=None, maxiter=None, window=200,
               debug_iter=None):
         self.op(RESIDUAL, b=b, x=x)
         s = self.read()
         bnorm = float(np.sqrt(s['bb']))
         tol = max(rtol * bnorm, atol) if atol else rtol * bnorm
         rnorm = float(np.sqrt(s['rr_true']))
-        fallback_at, why = None, None
+        fallback_at, why, rejected = None, None, 0
         it = 0
         if M is not None and not rnorm <= tol:
             best, best_it = rnorm, 0
@@ -162,8 +162,9 @@ class FcgKernels:
                     continue
                 if rnorm < best * 0.999:
                     best, best_it = rnorm, it
-                if s['nbad']:
-                    why = 'non-descent direction'
+                rejected += s['nbad']
+                if 2 * s['nbad'] > s['it']:
+                    why = 'non-descent directions'
                 elif it - best_it > window:
                     why = 'stagnation'
                 if why:
@@ -180,6 +181,7 @@ class FcgKernels:
                 if rnorm <= tol:
                     break
         return {'residual': rnorm, 'iters': it, 'fallback_at': fallback_at,
+                'rejected_steps': rejected,
                 'bnorm': bnorm}
 
 
@@ -224,7 +226,7 @@ class FlexibleCG(esr.Module):
                               f['n_grid'], self.x.device)
             self.easier_M.bind(graph)
 
-    def solve(self, rtol=1e-8, atol=None, maxiter=None, window=50,
+    def solve(self, rtol=1e-8, atol=None, maxiter=None, window=200,
               debug_iter=None) -> Dict[str, Any]:
         if self.easier_kernels is None:
             raise RuntimeError('run esr.compile() first')

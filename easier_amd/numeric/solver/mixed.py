@@ -21,6 +21,7 @@ from torch import nn
 
 from ... import _native as nat
 from ...core import module as esr
+from ...core import ops
 from ...core.utils import logger
 from .fused import LinsysCSR, _Graph, _precond_diag
 
@@ -30,6 +31,7 @@ nat.register_signatures({
     'esr_ir_residual': (_c_i32, [_c_i64] + [_c_vp] * 9 + [_c_i32, _c_vp]),
     'esr_ir_update': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
     'esr_ir_read': (_c_i32, [_c_vp, ctypes.POINTER(ctypes.c_double), _c_vp]),
+    'esr_pcg_rearm': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 6 + [ctypes.c_double, _c_vp]),
 })
 
 
@@ -120,10 +122,20 @@ class MixedPrecisionCG(esr.Module):
 
     def solve(self, rtol: float = 1e-10, atol: Optional[float] = None,
               maxiter: Optional[int] = None, inner_rtol: float = 1e-4,
-              max_outer: int = 50, chunk: int = 32,
+              max_outer: int = 200, chunk: int = 32, reliable: bool = True,
+              delta: float = 0.1,
               debug_iter: Optional[int] = None) -> Dict[str, Any]:
-        """Returns {'residual': true float64 ||b - A x||, 'iters': total inner
-        iterations, 'outer': refinement steps}. maxiter caps each inner solve."""
+        """Returns {'residual': true float64 ||b - A x||, 'iters': float32
+        steps, 'outer': float64 residual updates, 'bnorm'}.
+
+        reliable=True (default): one float32 CG whose correction is folded
+        into the float64 x, and whose residual is replaced by the true
+        float64 one, each time the recursive residual has dropped by `delta`
+        since the last update; the search direction is kept across updates
+        (reliable updates, Sleijpen and van der Vorst 1996), so the step count
+        stays close to float64 CG. maxiter caps the total float32 steps.
+        reliable=False: classic iterative refinement, a fresh float32 solve
+        to `inner_rtol` per outer step (maxiter caps each)."""
         if self.easier_state is None:
             raise RuntimeError('run esr.compile() first')
         s = self.easier_state
@@ -135,26 +147,47 @@ class MixedPrecisionCG(esr.Module):
         total, outer = 0, 0
         sc = (ctypes.c_double * 6)()
         it = (ctypes.c_int64 * 3)()
-        while not (rnorm == rnorm and rnorm <= tol) and outer < max_outer:
-            outer += 1
-            # inner float32 Jacobi-PCG on A d = r from d = 0
-            nat.call('esr_pcg_init', nat.F32, s['n'], *self._inner_arrays(),
-                     s['r32'].data_ptr(), s['d32'].data_ptr(), s['ri'].data_ptr(),
-                     s['z'].data_ptr(), s['p'].data_ptr(), s['pcg_state'].data_ptr(),
-                     s['pcg_ws'].data_ptr(), float(inner_rtol), 0.0,
-                     -1 if maxiter is None else int(maxiter), nat.stream_ptr())
+
+        def run_inner():
             while True:
                 s['graph'].replay()
                 nat.call('esr_pcg_read', s['pcg_state'].data_ptr(), sc, it,
                          nat.stream_ptr())
                 if it[1]:
+                    return
+
+        started = False
+        while not (rnorm == rnorm and rnorm <= tol) and outer < max_outer:
+            outer += 1
+            if reliable and started:
+                if maxiter is not None and it[0] >= maxiter:
                     break
-            total += it[0] + 1
+                # the inner residual is the true one again; keep p
+                ops.copy_(s['ri'], s['r32'])
+                nat.call('esr_pcg_rearm', nat.F32, s['n'],
+                         None if s['diag32'] is None else s['diag32'].data_ptr(),
+                         s['ri'].data_ptr(), s['z'].data_ptr(), s['p'].data_ptr(),
+                         s['pcg_state'].data_ptr(), s['pcg_ws'].data_ptr(),
+                         max(delta * rnorm, tol), nat.stream_ptr())
+            else:
+                # float32 Jacobi-PCG on A d = r from d = 0
+                inner_tol = max(delta * rnorm, tol) if reliable else inner_rtol * rnorm
+                nat.call('esr_pcg_init', nat.F32, s['n'], *self._inner_arrays(),
+                         s['r32'].data_ptr(), s['d32'].data_ptr(), s['ri'].data_ptr(),
+                         s['z'].data_ptr(), s['p'].data_ptr(), s['pcg_state'].data_ptr(),
+                         s['pcg_ws'].data_ptr(), 0.0, float(inner_tol),
+                         -1 if maxiter is None else int(maxiter), nat.stream_ptr())
+                started = True
+            run_inner()
+            if not reliable:
+                total += it[0] + 1
             nat.call('esr_ir_update', s['n'], self.x.data.data_ptr(),
                      s['d32'].data_ptr(), nat.stream_ptr())
             rnorm, _ = self._residual(False)
             if debug_iter is not None:
-                logger.info(f'MixedPrecisionCG outer {outer}: inner {it[0] + 1}, '
-                            f'true residual {rnorm}')
+                logger.info(f'MixedPrecisionCG update {outer}: float32 steps '
+                            f'{it[0] + 1}, true residual {rnorm}')
+        if reliable:
+            total = it[0] + 1 if started else 0
         return {'residual': rnorm, 'iters': total, 'outer': outer,
                 'bnorm': bnorm}

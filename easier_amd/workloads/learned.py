@@ -137,22 +137,64 @@ def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=
     return out
 
 
+def generate_pairs_fcg(prob: Problem, pre: LearnedPreconditioner, n_rhs, seed,
+                       every=32, max_snaps=24, store_dtype=torch.float32):
+    """Pairs from the iterates of flexible CG preconditioned by the network
+    itself (the residuals the network meets in use): the same right-hand
+    sides as generate_pairs' seed, a snapshot every `every` steps."""
+    from ..numeric.solver.fcg import RESIDUAL as RESIDUAL_OP, FcgKernels
+    fk = FcgKernels(prob.rowptr, prob.col, prob.Ae, prob.Av, prob.n, prob.device,
+                    chunk=every)
+    pre.bind(prob.graph)
+    pcg = _Pcg(prob)
+    out = []
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.zeros(prob.n, dtype=torch.float64, device=prob.device)
+    for k in range(n_rhs):
+        b = (torch.rand(prob.n, generator=gen, dtype=torch.float64) * 2 - 1)
+        b[torch.from_numpy(prob.sys['boundary'])] = 0.0
+        bd = b.to(prob.device)
+        ops.fill_(x, 0)
+        fk.op(RESIDUAL_OP, b=bd, x=x)
+        tol = 1e-10 * float(np.sqrt(fk.read()['bb']))
+        nat.call('esr_fcg_set', fk.state.data_ptr(), tol * tol, -1, nat.stream_ptr())
+        snaps = [(fk.r.clone(), x.to(store_dtype))]
+        fk._step(pre, x, True)
+        while len(snaps) < max_snaps:
+            fk._chunk_graph(pre, x).replay()
+            s = fk.read()
+            if s['done']:
+                break
+            snaps.append((fk.r.clone(), x.to(store_dtype)))
+        pcg.init(bd, 1e-10, maxiter=max(20000, 4 * prob.n_grid * prob.n_grid))
+        while not pcg.read()['done']:
+            pcg.steps(256)
+        xstar = pcg.vec['x']
+        for r, xs in snaps:
+            e = ops.ewise('sub', xstar, xs.to(torch.float64))
+            en2 = _dot(e, e)
+            if en2 > 0:
+                out.append((r, e.to(store_dtype), en2, _dot(r, e)))
+    return out
+
+
 def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
     """(loss, dloss/dz) of the preconditioned vector z = M(r).
 
     kind 'error':  ||z - e||^2 / ||e||^2.
-    kind 'energy': 1 - (z.r)^2 / ((z.Az) (r.e)) -- the fraction of the
-        A-norm error left after the best step along z (r.e = ||e||_A^2); its
-        gradient is -2 (z.r) / ((z.Az)(r.e)) (r - (z.r)/(z.Az) A z).
+    kind 'energy': 1 - (z.r)|z.r| / ((z.Az) (r.e)) -- for z.r > 0 the
+        fraction of the A-norm error left after the best step along z
+        (r.e = ||e||_A^2); above 1 when z is not a descent direction. Its
+        gradient is -2 |z.r| / ((z.Az)(r.e)) (r - (z.r)/(z.Az) A z).
         Scale-invariant in z, like the step flexible CG takes."""
     if kind == 'error':
         diff = ops.ewise('sub', z, e)
         return _dot(diff, diff) / en2, ops.ewise('mul', diff, 2.0 / en2)
     q = spmv(prob, z)
     zr, zq = _dot(z, r), _dot(z, q)
-    loss = 1.0 - zr * zr / (zq * re)
+    loss = 1.0 - zr * abs(zr) / (zq * re)
     t = ops.ewise('sub', r, ops.ewise('mul', q, zr / zq))
-    return loss, ops.ewise('mul', t, -2.0 * zr / (zq * re))
+    return loss, ops.ewise('mul', t, -2.0 * abs(zr) / (zq * re))
 
 
 def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every=200,
@@ -275,6 +317,9 @@ def main(argv=None):
     tr.add_argument('--layers', type=int, default=3)
     tr.add_argument('--seed0', type=int, default=1000)
     tr.add_argument('--loss', choices=['energy', 'error'], default='energy')
+    tr.add_argument('--policy-rounds', type=int, default=1,
+                    help='rounds of pairs from flexible CG with the network itself')
+    tr.add_argument('--policy-rhs', type=int, default=8)
     tr.add_argument('--out', required=True)
     ev = sub.add_parser('evaluate', help='learned FCG vs Jacobi-PCG on unseen seeds')
     ev.add_argument('--weights', required=True)
@@ -296,14 +341,26 @@ def main(argv=None):
         base = mean_loss(None, data, args.loss)
         _log(f'{npairs} pairs; Jacobi {args.loss} loss {base:.4f}')
         pre = LearnedPreconditioner(args.layers, lp.init_params(args.layers, args.seed0))
-        for ep in range(args.epochs):
-            lr = args.lr * (0.5 ** ep)
-            losses = train(pre, data, epochs=1, lr=lr, seed=ep, log_every=1000,
-                           loss=args.loss)
-            _log(f'epoch {ep + 1}/{args.epochs} lr {lr:.2e}: mean {losses[-1][1]:.4f}; ' +
-                 ', '.join(f'{s}:{l:.4f}' for s, l in losses[-4:-1]) +
-                 f' ({time.perf_counter() - t0:.1f} s)')
+
+        def epochs(tag):
+            for ep in range(args.epochs):
+                lr = args.lr * (0.5 ** ep)
+                losses = train(pre, data, epochs=1, lr=lr, seed=ep, log_every=1000,
+                               loss=args.loss)
+                _log(f'{tag} epoch {ep + 1}/{args.epochs} lr {lr:.2e}: mean '
+                     f'{losses[-1][1]:.4f} ({time.perf_counter() - t0:.1f} s)')
+        epochs('pcg pairs')
+        for rnd in range(args.policy_rounds):
+            # residuals of flexible CG with the current network, added to the set
+            for k, (prob, pairs) in enumerate(data):
+                pairs.extend(generate_pairs_fcg(prob, pre, args.policy_rhs,
+                                                seed=10 ** 6 + args.seed0 + 97 * rnd + k))
+            npairs = sum(len(p) for _, p in data)
+            _log(f'round {rnd + 1}: {npairs} pairs with flexible-CG iterates '
+                 f'({time.perf_counter() - t0:.1f} s)')
+            epochs(f'round {rnd + 1}')
         final = mean_loss(pre, data, args.loss)
+        base = mean_loss(None, data, args.loss)
         _log(f'trained {args.loss} loss {final:.4f} (Jacobi {base:.4f})')
         lp.save(args.out, pre, n_train=args.n, pairs=npairs, jacobi_loss=base,
                 trained_loss=final, loss=args.loss,

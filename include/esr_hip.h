@@ -256,6 +256,12 @@ int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr,
                   const void *diag, void *x, void *r, void *z, void *p,
                   void *Ap, void *state, void *ws, int64_t count,
                   void *stream);
+/* Continue a stopped PCG after the caller replaced r by a freshly computed
+ * residual (a reliable update of mixed-precision CG; no reference
+ * counterpart): z = M r, beta = r.z / rz, p = z + beta p, new absolute exit
+ * tolerance; the step counter continues.                                     */
+int esr_pcg_rearm(int dtype, int64_t n, const void *diag, const void *r, void *z,
+                  void *p, void *state, void *ws, double tol, void *stream);
 int esr_pcg_read(const void *state, double *scalars, int64_t *ints,
                  void *stream);
 
@@ -323,6 +329,20 @@ int esr_bcg_steps(int R, int64_t ntiles, const int32_t *tinst,
                   double *z, double *p, double *Ap, void *state, double *part,
                   int32_t *active, int64_t count, void *stream);
 
+/* ---- K <= 64 right-hand sides of one operator (cg.py:12-137 per
+ * right-hand side, new form of the batched solver): vectors (n, K) row-major,
+ * per-right-hand-side state (the batched solver's layout, rtol/atol/maxiter
+ * set by the caller), the operator read once per iteration.                 */
+int esr_mrhs_sizes(int64_t n, int64_t K, int64_t *state_bytes, int64_t *ws_bytes);
+int esr_mrhs_init(int64_t n, int64_t K, const int32_t *rowptr, const int32_t *col,
+                  const double *Ae, const double *Av, const double *diag,
+                  const double *B, const double *X, double *R, double *Z, double *P,
+                  void *state, double *ws, int32_t *active, void *stream);
+int esr_mrhs_steps(int64_t n, int64_t K, const int32_t *rowptr, const int32_t *col,
+                   const double *Ae, const double *Av, const double *diag, double *X,
+                   double *R, double *Z, double *P, double *AP, void *state, double *ws,
+                   int32_t *active, int64_t count, void *stream);
+
 /* ---- learned preconditioner (new; the reference's preconditioner slot,
  * easier/numeric/preconditioner/__init__.py:4, is empty) --------------------
  * z = M(r), an edge message-passing network with L layers of width 32 (see
@@ -357,7 +377,9 @@ int esr_gnn_adam(int64_t np_, float *params, const float *grad, float *m,
 /* ---- flexible CG (Polak-Ribiere) steps for a varying preconditioner ------
  * op: 0 q = A p, p.q, alpha = rz / p.q; 1 r_old = r, r -= alpha q,
  * x += alpha p, ||r||^2; 2 z.r, z.r_old, beta (a0 != 0: restart, beta = 0),
- * rz = z.r; 3 p = z + beta p; 4 r = b - A x, ||r||^2, ||b||^2; 5 z = r / d.
+ * rz = z.r -- a z with z.r <= 0 or not finite is replaced by r / d for the
+ * step; 3 p = z + beta p (a0 != 0: p = z); 4 r = b - A x, ||r||^2, ||b||^2;
+ * 5 z = r / d.
  * esr_fcg_set arms the step operations (0, 1, 2, 3, 5): they do nothing once
  * a step has brought ||r||^2 to tol2 or maxit steps were taken (maxit < 0:
  * no limit), so chunks of steps can be captured in a graph and replayed.

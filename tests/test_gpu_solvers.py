@@ -45,16 +45,29 @@ def test_batched_uneven_meshes_match_individual_solves(gpu):
         assert np.array_equal(a, b)
 
 
-def test_batched_many_rhs_one_mesh(gpu):
+@pytest.mark.parametrize('K', [8, 3, 64])
+def test_batched_many_rhs_one_mesh(gpu, K):
     mesh = wmesh.tri2(24)
-    base = wp.p1_system(mesh, None, None)
     systems = []
-    for k in range(8):
+    for k in range(K):
         f = wp.rhs_uniform(mesh['points'].shape[0], k)
         systems.append(wp.p1_system(mesh, None, f))
-    bcg = BatchedCG(systems)
+    bcg = BatchedCG(systems)                       # shared operator
+    assert bcg.multi is not None
     infos = bcg.solve(rtol=1e-8)
-    for s, info in zip(systems, infos):
-        info_o, _ = _oracle_solve(s, 1e-8)
+    xs = bcg.solutions()
+    sep = BatchedCG(systems, share_operator=False)
+    infos_sep = sep.solve(rtol=1e-8)
+    xs_sep = sep.solutions()
+    for k in (0, K // 2, K - 1):
+        s, info = systems[k], infos[k]
+        info_o, x_o = _oracle_solve(s, 1e-8)
         assert abs(info['iters'] - info_o['iters']) <= 1
-    assert base['Ae'].shape == systems[0]['Ae'].shape
+        assert abs(info['iters'] - infos_sep[k]['iters']) <= 1
+        assert info['residual'] <= 1e-8 * np.linalg.norm(s['b']) * 1.0000001
+        np.testing.assert_allclose(xs[k], x_o, rtol=1e-6, atol=1e-9)
+        np.testing.assert_allclose(xs[k], xs_sep[k], rtol=1e-6, atol=1e-9)
+    # replaying the captured iterations again gives the same bits
+    assert bcg.solve(rtol=1e-8) == infos
+    for a, b in zip(xs, bcg.solutions()):
+        assert np.array_equal(a, b)
