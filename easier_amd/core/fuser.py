@@ -61,10 +61,68 @@ class Unsupported(Exception):
 #                  stream through neighbouring rows together, which keeps the
 #                  ~20 concurrent array streams of a stage kernel on fewer
 #                  DRAM pages (measured faster for the SWE stage kernels).
+#   sparse_rows  - a Reducer with few non-empty rows (at most 1 in 8) finds a
+#                  row's entries through a bitmap of the non-empty rows (one
+#                  64-bit word and one prefix count per 64 rows) and the row
+#                  pointers of the non-empty rows only, instead of reading
+#                  two row pointers for every row.
 # Defaults are the fastest measured combination for the 2048 x 2048 SWE step.
+#   idx16        - in dense Reducer groups, a Selector's index is read as int16
+#                  offsets from the edge's row when all of them fit (a mesh
+#                  renumbered for locality), half the bytes of int32 indices.
+#   xcd_group    - G > 0: consecutive groups of G tiles per XCD (esr_group_remap);
+#                  overrides xcd_remap. Measured on the 2048 x 2048 SWE step
+#                  (bytes fetched from beyond L2 / algorithmic, ms per step):
+#                  no remap 1.285 / 1.189, xcd_remap 1.002 / 1.217,
+#                  G = 16 1.105 / 1.164, G = 64 1.035 / 1.175, G = 256
+#                  1.009 / 1.196.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
-           'xcd_remap': False}
+           'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64}
 _LDS_BUDGET = 48 * 1024
+
+
+def _delta16(sel, red):
+    """int16 device array idx[e] - row(e) for the edges of a Reducer-ordered
+    edge space (row(e) the Reducer row of edge e), or None when an offset
+    does not fit in int16."""
+    cache = sel.__dict__.setdefault('easier_delta16', {})
+    if id(red) in cache:
+        return cache[id(red)]
+    hp = np.asarray(red.easier_host_rowptr, np.int64)
+    idx = np.asarray(sel.easier_host_idx, np.int64)
+    rows = np.repeat(np.arange(hp.shape[0] - 1, dtype=np.int64), np.diff(hp))
+    out = None
+    if rows.shape == idx.shape:
+        d = idx - rows
+        if d.size == 0 or (d.min() >= -32768 and d.max() <= 32767):
+            out = torch.from_numpy(d.astype(np.int16)).to(red.easier_rowptr.device)
+    cache[id(red)] = out
+    return out
+
+
+def _sparse_rows(red):
+    """(bitmap, prefix, compact row pointers) of a Reducer's non-empty rows,
+    built once from its host CSR: bit r % 64 of word r // 64 is set when row
+    r has entries; prefix[w] counts the set bits of words before w; the c-th
+    non-empty row's entries are [crow[c], crow[c + 1])."""
+    cached = getattr(red, 'easier_sparse_rows', None)
+    if cached is not None:
+        return cached
+    hp = np.asarray(red.easier_host_rowptr, np.int64)
+    n = hp.shape[0] - 1
+    nonempty = np.nonzero(hp[1:] > hp[:-1])[0]
+    nw = (n + 63) // 64
+    bits = np.zeros((nw, 64), np.uint64)
+    bits.reshape(-1)[nonempty] = 1
+    words = (bits << np.arange(64, dtype=np.uint64)).sum(1, dtype=np.uint64)
+    prefix = np.concatenate([[0], np.cumsum(bits.sum(1, dtype=np.int64))[:-1]])
+    crow = np.concatenate([hp[nonempty], [hp[-1]]])
+    dev = red.easier_rowptr.device
+    out = (torch.from_numpy(words.view(np.int64)).to(dev),
+           torch.from_numpy(prefix.astype(np.int32)).to(dev),
+           torch.from_numpy(crow.astype(np.int32)).to(dev))
+    red.easier_sparse_rows = out
+    return out
 
 
 _EW_BIN = {
@@ -348,6 +406,20 @@ def _fmt_const(x: float, ctype: str) -> str:
 _CODE_HEADER = r'''
 typedef long long i64;
 #define ESR_XCD 8
+__device__ __forceinline__ int esr_clampi(int v, int hi) {
+  return v < 0 ? 0 : (v > hi ? hi : v);
+}
+// Workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th: groups of G
+// consecutive tiles go to one XCD, XCDs taking consecutive groups, so the 8
+// XCDs stream through one window of 8 G tiles together while a tile's
+// neighbours mostly stay in its XCD's L2. Tiles past the last whole window
+// keep their own index (a bijection on [0, nb)).
+__device__ __forceinline__ i64 esr_group_remap(i64 b, i64 nb, i64 G) {
+  const i64 full = (nb / (8 * G)) * 8 * G;
+  if (b >= full) return b;
+  const i64 x = b % 8, s = b / 8;
+  return ((s / G) * 8 + x) * G + (s % G);
+}
 __device__ __forceinline__ i64 esr_remap(i64 b, i64 nb) {
   const i64 q = nb / ESR_XCD, r = nb % ESR_XCD;
   const i64 x = b % ESR_XCD, pos = b / ESR_XCD;
@@ -593,15 +665,25 @@ class _Lowering:
             members = [u for u in self.p.values if u.kind == 'reduce' and u.obj is red
                        and u.stage == kern.stage and u.space == kern.space
                        and (u.id in needed)]
-            rp = self._param(kern, red.easier_rowptr, 'int')
-            perm = None if red.easier_perm is None else \
-                self._param(kern, red.easier_perm, 'int')
-            body: List[str] = []
-            emem: Dict[int, str] = {}
             hp = red.easier_host_rowptr
             counts = np.diff(hp)
             kmax = int(counts.max()) if counts.size else 0
             dense = 0 < kmax <= 8 and counts.mean() >= 0.5 * kmax
+            # few non-empty rows (wall faces of a mesh): a bitmap of the
+            # non-empty rows replaces the row pointers
+            sparse = (not dense and OPTIONS['sparse_rows'] and counts.size >= 4096 and
+                      np.count_nonzero(counts) * 8 <= counts.size)
+            if sparse:
+                bm, pre, crow = _sparse_rows(red)
+                bmp = self._param(kern, bm, 'unsigned long long')
+                prp = self._param(kern, pre, 'int')
+                rp = self._param(kern, crow, 'int')
+            else:
+                rp = self._param(kern, red.easier_rowptr, 'int')
+            perm = None if red.easier_perm is None else \
+                self._param(kern, red.easier_perm, 'int')
+            body: List[str] = []
+            emem: Dict[int, str] = {}
             gid = len(done_groups)
             # edge arrays staged through LDS: param name -> (lds name, ctype)
             staged: Dict[str, tuple] = {}
@@ -618,6 +700,28 @@ class _Lowering:
                     return f'__builtin_nontemporal_load(&{name}[e])'
                 return f'{name}[e]'
 
+            gidx: Dict[int, str] = {}
+
+            def gather_index(sel, src):
+                """The Selector's source row of edge e, loaded once per edge:
+                int16 offsets from the edge's own row when they fit (dense
+                rows in Reducer order), clamped into the source as the
+                clamped slots of an empty row may belong to another row."""
+                if id(sel) in gidx:
+                    return gidx[id(sel)]
+                d = _delta16(sel, red) if (dense and perm is None and not stage and
+                                           OPTIONS['idx16']) else None
+                if d is not None:
+                    dn = self._param(kern, d, 'short')
+                    hi = self.prog.space_len[src.space] - 1
+                    s = var(body, f'esr_clampi((int)r + (int){edge_load(dn, "short")}, {hi})',
+                            'int')
+                else:
+                    idx = self._param(kern, sel.idx, 'int')
+                    s = var(body, edge_load(idx, 'int'), 'int')
+                gidx[id(sel)] = s
+                return s
+
             def emit_edge(x: V) -> str:
                 if x.id in emem:
                     return emem[x.id]
@@ -631,8 +735,7 @@ class _Lowering:
                         s = emit_row(src) if src.kind != 'leaf' else \
                             emit_row_leaf(src)
                     else:
-                        idx = self._param(kern, x.obj.idx, 'int')
-                        s = var(body, f'{buf_of(src)}[{edge_load(idx, "int")}]')
+                        s = var(body, f'{buf_of(src)}[{gather_index(x.obj, src)}]')
                 elif x.kind == 'ew':
                     a = [emit_edge(y) for y in x.args]
                     s = var(body, ew_expr(x, a), 'bool' if is_pred(x) else None)
@@ -692,6 +795,18 @@ class _Lowering:
                     rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
                 rows.extend('  ' + b for b in body)
                 rows.append('  } }')
+            elif sparse:
+                # compact row c of the row: the set bits of the bitmap before it
+                for u, a in zip(members, accs):
+                    body.append(f'  {a} = {a} + {inner[members.index(u)]};')
+                rows.append(f'  {{ const unsigned long long bw{gid} = {bmp}[r >> 6];')
+                rows.append(f'  if ((bw{gid} >> (r & 63)) & 1ull) {{')
+                rows.append(f'    const int c = {prp}[r >> 6] + '
+                            f'__popcll(bw{gid} & ((1ull << (r & 63)) - 1ull));')
+                rows.append(f'    for (int j = {rp}[c]; j < {rp}[c + 1]; ++j) {{')
+                rows.append(f'      const int e = {perm + "[j]" if perm else "j"};')
+                rows.extend('    ' + b for b in body)
+                rows.append('    } } }')
             else:
                 for u, a in zip(members, accs):
                     body.append(f'  {a} = {a} + {inner[members.index(u)]};')
@@ -738,8 +853,12 @@ class _Lowering:
             st = [f'  {d}[r] = {s};' for d, s in stores]
         params = ''.join(f'  {ctyp} __restrict__ p{i};\n'
                          for i, (_, ctyp, _) in enumerate(kern.params))
-        tile = 'esr_remap(blockIdx.x, gridDim.x)' if OPTIONS['xcd_remap'] else \
-            '(i64)blockIdx.x'
+        if OPTIONS['xcd_group']:
+            tile = f'esr_group_remap(blockIdx.x, gridDim.x, {int(OPTIONS["xcd_group"])})'
+        elif OPTIONS['xcd_remap']:
+            tile = 'esr_remap(blockIdx.x, gridDim.x)'
+        else:
+            tile = '(i64)blockIdx.x'
         if kern.staged:
             # every thread of the block takes part in the LDS staging: rows
             # past the end recompute the last row and do not store

@@ -641,6 +641,49 @@ ESR_API int esr_aggregate(int op, int dtype, const void *x, int64_t n, int64_t w
   ESR_GUARD_END
 }
 
+namespace esr {
+namespace {
+// Dense 1-d copy / add / sub / mul / div in one floating dtype: one element
+// per thread, inputs with stride 1, stride 0 (a one-element tensor) or a
+// scalar (null pointer). The streaming case of the general kernel above
+// without its per-element shape walk and operator switch.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_dense(int64_t n, const T *__restrict__ a, int64_t sa,
+                                                 T ca, const T *__restrict__ b, int64_t sb, T cb,
+                                                 T *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T x = a ? a[i * sa] : ca;
+  if constexpr (OP == ESR_EW_COPY) {
+    out[i] = x;
+  } else {
+    const T y = b ? b[i * sb] : cb;
+    if constexpr (OP == ESR_EW_ADD) out[i] = x + y;
+    else if constexpr (OP == ESR_EW_SUB) out[i] = x - y;
+    else if constexpr (OP == ESR_EW_MUL) out[i] = x * y;
+    else out[i] = x / y;
+  }
+}
+
+template <typename T>
+int launch_dense(const EwParams &p, void *out, hipStream_t s) {
+  const unsigned g = (unsigned)ceil_div(p.total, kBlock);
+  const T *a = (const T *)p.in[0], *b = (const T *)p.in[1];
+  const int64_t sa = p.istr[0][0], sb = p.istr[1][0];
+  const T ca = (T)p.sc[0], cb = (T)p.sc[1];
+  switch (p.op) {
+  case ESR_EW_COPY: k_dense<T, ESR_EW_COPY><<<g, kBlock, 0, s>>>(p.total, a, sa, ca, b, sb, cb, (T *)out); break;
+  case ESR_EW_ADD: k_dense<T, ESR_EW_ADD><<<g, kBlock, 0, s>>>(p.total, a, sa, ca, b, sb, cb, (T *)out); break;
+  case ESR_EW_SUB: k_dense<T, ESR_EW_SUB><<<g, kBlock, 0, s>>>(p.total, a, sa, ca, b, sb, cb, (T *)out); break;
+  case ESR_EW_MUL: k_dense<T, ESR_EW_MUL><<<g, kBlock, 0, s>>>(p.total, a, sa, ca, b, sb, cb, (T *)out); break;
+  default: k_dense<T, ESR_EW_DIV><<<g, kBlock, 0, s>>>(p.total, a, sa, ca, b, sb, cb, (T *)out); break;
+  }
+  ESR_LAUNCHED();
+  return ESR_OK;
+}
+} // namespace
+} // namespace esr
+
 ESR_API int esr_ewise(int op, int ndim, const int64_t *shape, int out_dtype, void *out,
                       const int64_t *out_strides, int in_dtype, int n_in,
                       void *const *in_ptrs, const int64_t *in_strides, const double *scalars,
@@ -683,6 +726,12 @@ ESR_API int esr_ewise(int op, int ndim, const int64_t *shape, int out_dtype, voi
   for (int k = 0; k < n_in && p.contiguous; ++k)
     if (p.in[k] && p.istr[k][0] != 1 && p.istr[k][0] != 0) p.contiguous = 0;
   hipStream_t s = as_stream(stream);
+  const bool dense_op = op == ESR_EW_COPY || op == ESR_EW_ADD || op == ESR_EW_SUB ||
+                        op == ESR_EW_MUL || op == ESR_EW_DIV;
+  if (p.contiguous && dense_op && in_dtype == out_dtype && p.total < (1ll << 40)) {
+    if (in_dtype == ESR_F64) return launch_dense<double>(p, out, s);
+    if (in_dtype == ESR_F32) return launch_dense<float>(p, out, s);
+  }
   if (op == ESR_EW_WHERE) {
     // condition is u8; value inputs in_dtype
     switch (in_dtype) {

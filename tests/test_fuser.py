@@ -3,6 +3,7 @@ them, and that the generated kernels compile for gfx950 (hipRTC, no device).
 """
 import ctypes
 
+import numpy as np
 import pytest
 import torch
 
@@ -44,10 +45,46 @@ def test_swe_step_fuses_into_four_kernels(dtype, kind):
     _rtc_ok(src)
 
 
+@pytest.mark.parametrize('nb,G', [(1, 16), (127, 4), (4096, 16), (4100, 64), (32768, 256)])
+def test_group_remap_is_a_permutation(nb, G):
+    # esr_group_remap of the generated code, evaluated for every workgroup
+    full = (nb // (8 * G)) * 8 * G
+    tiles = [b if b >= full else ((b // 8 // G) * 8 + b % 8) * G + (b // 8) % G
+             for b in range(nb)]
+    assert sorted(tiles) == list(range(nb))
+    assert f'esr_group_remap(i64 b, i64 nb, i64 G)' in fuser._CODE_HEADER
+
+
+def test_sparse_row_bitmap_finds_every_row():
+    class _Red:
+        pass
+    rng = np.random.default_rng(5)
+    n = 5000
+    counts = np.where(rng.random(n) < 0.05, rng.integers(1, 4, n), 0)
+    red = _Red()
+    red.easier_host_rowptr = np.concatenate([[0], np.cumsum(counts)])
+    red.easier_rowptr = torch.zeros(1, dtype=torch.int32)
+    bm, pre, crow = (t.numpy() for t in fuser._sparse_rows(red))
+    words = bm.view(np.uint64)
+    for r in range(n):
+        w, b = r >> 6, r & 63
+        present = bool((int(words[w]) >> b) & 1)
+        assert present == (counts[r] > 0)
+        if present:      # what the generated kernel computes
+            c = int(pre[w]) + bin(int(words[w]) & ((1 << b) - 1)).count('1')
+            assert (crow[c], crow[c + 1]) == (red.easier_host_rowptr[r],
+                                              red.easier_host_rowptr[r + 1])
+
+
 @pytest.mark.parametrize('opts', [
-    dict(stage_edges=False, nt_loads=False, nt_stores=False, block=256, xcd_remap=True),
-    dict(stage_edges=True, nt_loads=False, nt_stores=True, block=128, xcd_remap=False),
-    dict(stage_edges=True, nt_loads=True, nt_stores=False, block=512, xcd_remap=True)])
+    dict(stage_edges=False, nt_loads=False, nt_stores=False, block=256, xcd_remap=True,
+         xcd_group=0),
+    dict(stage_edges=True, nt_loads=False, nt_stores=True, block=128, xcd_remap=False,
+         xcd_group=0),
+    dict(stage_edges=True, nt_loads=True, nt_stores=False, block=512, xcd_remap=True,
+         xcd_group=0),
+    dict(stage_edges=False, nt_loads=False, nt_stores=True, block=256, xcd_remap=False,
+         xcd_group=64)])
 def test_code_generation_options(opts):
     mesh = wmesh.tri2(7)
     geo = oswe.geometry(mesh)
@@ -63,7 +100,8 @@ def test_code_generation_options(opts):
     assert all(k.staged == opts['stage_edges'] for k in low.kernels)
     assert ('__syncthreads' in src) == opts['stage_edges']
     assert ('__builtin_nontemporal_store' in src) == opts['nt_stores']
-    assert ('esr_remap(blockIdx.x' in src) == opts['xcd_remap']
+    assert ('esr_remap(blockIdx.x' in src) == (opts['xcd_remap'] and not opts['xcd_group'])
+    assert ('esr_group_remap(blockIdx.x' in src) == bool(opts['xcd_group'])
     # the staged edge range of one block fits its LDS arrays: 3 edges per row
     for k in low.kernels:
         assert k.lds_bytes <= fuser._LDS_BUDGET
