@@ -1,0 +1,420 @@
+// Geometric multigrid on the 2-triangle mesh: the hierarchy and the V-cycle
+// kernels of the multilevel preconditioner used inside flexible CG.
+// No reference counterpart: the reference's preconditioner slot
+// (easier/numeric/preconditioner/__init__.py:4) is empty and its solvers take
+// any callable M (numeric/solver/cg.py:86).
+//
+// Grids. Level l is the (n+1) x (n+1) vertex grid of the 2-triangle mesh
+// (squares split along the SW-NE diagonal); vertex (I, J) sits at
+// lexicographic position J (n+1) + I. Coarse levels are stored in
+// lexicographic order; the finest level may be in any order, given by
+// fpos (fine index -> position) and fidx (position -> fine index).
+//
+// Transfer. P is P1 interpolation between the nested meshes: a fine vertex
+// at an even-even position copies its coarse parent, one on a horizontal,
+// vertical or SW-NE coarse edge takes the mean of the edge's two ends, and
+// coarse boundary vertices (zero Dirichlet values) contribute nothing.
+// Restriction is P^T, gathered per coarse vertex over its 7 children in a
+// fixed order (no atomics, deterministic).
+//
+// Coarse operators are Galerkin products P^T A P on the coarse mesh's vertex
+// graph (7-point stencil: SW, S, W, E, N, NE + the diagonal), one thread per
+// coarse vertex summing its children's rows in a fixed order; boundary rows
+// are identity rows, as in the P1 assembly.
+#include "esr_common.h"
+#include "esr_krylov.h"
+
+namespace esr {
+namespace {
+
+constexpr int kMB = 256;
+
+__device__ __forceinline__ int64_t mg_tid() { return (int64_t)blockIdx.x * kMB + threadIdx.x; }
+
+// (P zc) at fine position p of an nf-grid; coarse boundary values read as 0.
+template <typename T>
+__device__ __forceinline__ T prolong_value(int64_t p, int64_t nf, const T *__restrict__ zc) {
+  const int64_t Wf = nf + 1, nc = nf >> 1, Wc = nc + 1;
+  const int64_t I = p % Wf, J = p / Wf;
+  auto Z = [&](int64_t a, int64_t b) -> T {
+    return (a > 0 && b > 0 && a < nc && b < nc) ? zc[b * Wc + a] : T(0);
+  };
+  const int64_t a = I >> 1, b = J >> 1;
+  if (!(I & 1) && !(J & 1)) return Z(a, b);
+  if (!(J & 1)) return T(0.5) * (Z(a, b) + Z(a + 1, b));
+  if (!(I & 1)) return T(0.5) * (Z(a, b) + Z(a, b + 1));
+  return T(0.5) * (Z(a, b) + Z(a + 1, b + 1));
+}
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+// Vertex graph of the n-grid in lexicographic order, neighbours ascending
+// (SW, S, W, E, N, NE where they exist). rowptr in closed form: a row of the
+// grid holds (2W-1)([y>0] + [y<W-1]) + 2(W-1) entries.
+__global__ __launch_bounds__(kMB) void k_mg_pattern(int64_t n, int32_t *__restrict__ rowptr,
+                                                   int32_t *__restrict__ col) {
+  const int64_t W = n + 1, v = mg_tid();
+  if (v >= W * W) return;
+  const int64_t x = v % W, y = v / W;
+  const int64_t ym = y > 0 ? y - 1 : 0, xm = x > 0 ? x - 1 : 0;
+  const int64_t full = (2 * W - 1) * (ym + y) + 2 * (W - 1) * y;
+  const int64_t part = (y > 0 ? xm + x : 0) + xm + x + (y < W - 1 ? 2 * x : 0);
+  int64_t k = full + part;
+  rowptr[v] = (int32_t)k;
+  if (v == W * W - 1) rowptr[W * W] = (int32_t)((W - 1) * (6 * W - 2));
+  const int dx[6] = {-1, 0, -1, 1, 0, 1}, dy[6] = {-1, -1, 0, 0, 1, 1};
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int64_t xx = x + dx[s], yy = y + dy[s];
+    if (xx >= 0 && xx < W && yy >= 0 && yy < W) col[k++] = (int32_t)(yy * W + xx);
+  }
+}
+
+// Slot of coarse offset (dx, dy): 0 = diagonal, 1..6 = SW, S, W, E, N, NE;
+// -1 outside the stencil.
+__device__ __forceinline__ int stencil_slot(int64_t dx, int64_t dy) {
+  if (dx == 0 && dy == 0) return 0;
+  if (dx == -1 && dy == -1) return 1;
+  if (dx == 0 && dy == -1) return 2;
+  if (dx == -1 && dy == 0) return 3;
+  if (dx == 1 && dy == 0) return 4;
+  if (dx == 0 && dy == 1) return 5;
+  if (dx == 1 && dy == 1) return 6;
+  return -1;
+}
+
+__global__ __launch_bounds__(kMB) void k_mg_galerkin(
+    int64_t nf, const int32_t *__restrict__ fidx, const int32_t *__restrict__ fpos,
+    const int32_t *__restrict__ f_rowptr, const int32_t *__restrict__ f_col,
+    const double *__restrict__ f_Ae, const double *__restrict__ f_Av,
+    const int32_t *__restrict__ c_rowptr, double *__restrict__ c_Ae, double *__restrict__ c_Av,
+    int32_t *__restrict__ bad) {
+  const int64_t nc = nf >> 1, Wc = nc + 1, Wf = nf + 1, c = mg_tid();
+  if (c >= Wc * Wc) return;
+  const int64_t a = c % Wc, b = c / Wc;
+  const int32_t cb = c_rowptr[c], ce = c_rowptr[c + 1];
+  if (a == 0 || b == 0 || a == nc || b == nc) {
+    c_Av[c] = 1.0;
+    for (int32_t e = cb; e < ce; ++e) c_Ae[e] = 0.0;
+    return;
+  }
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  // children in the restriction's order: self, W, E, S, N, SW, NE
+  const int cdx[7] = {0, -1, 1, 0, 0, -1, 1}, cdy[7] = {0, 0, 0, -1, 1, -1, 1};
+  int nbad = 0;
+  for (int k = 0; k < 7; ++k) {
+    const double wk = k == 0 ? 1.0 : 0.5;
+    const int64_t p = (2 * b + cdy[k]) * Wf + 2 * a + cdx[k];
+    const int64_t i = fidx ? fidx[p] : p;
+    const int32_t rb = f_rowptr[i], re = f_rowptr[i + 1];
+    for (int32_t e = rb - 1; e < re; ++e) {   // e = rb - 1: the diagonal entry
+      const int64_t j = e < rb ? i : f_col[e];
+      const double v = wk * (e < rb ? f_Av[i] : f_Ae[e]);
+      if (v == 0.0) continue;
+      const int64_t q = fpos ? fpos[j] : j;
+      const int64_t I = q % Wf, J = q / Wf, pa = I >> 1, pb = J >> 1;
+      // parents of j (weights 1 or 1/2), interior ones only
+      int64_t ax[2], by[2];
+      double w;
+      int np = 2;
+      ax[0] = pa; by[0] = pb;
+      if (!(I & 1) && !(J & 1)) { np = 1; w = 1.0; }
+      else if (!(J & 1)) { ax[1] = pa + 1; by[1] = pb; w = 0.5; }
+      else if (!(I & 1)) { ax[1] = pa; by[1] = pb + 1; w = 0.5; }
+      else { ax[1] = pa + 1; by[1] = pb + 1; w = 0.5; }
+      for (int s = 0; s < np; ++s) {
+        if (ax[s] <= 0 || by[s] <= 0 || ax[s] >= nc || by[s] >= nc) continue;
+        const int slot = stencil_slot(ax[s] - a, by[s] - b);
+        if (slot < 0) { ++nbad; continue; }
+        acc[slot] += v * w;
+      }
+    }
+  }
+  if (nbad) atomicAdd(bad, nbad);
+  c_Av[c] = acc[0];
+  // interior coarse vertices hold all six neighbours, ascending
+  for (int s = 0; s < 6 && cb + s < ce; ++s) c_Ae[cb + s] = acc[s + 1];
+}
+
+// Network inputs of a lexicographic level (the learned smoother):
+// vfeat = (log(d / mean d), log kappa), efeat = (a_ij / sqrt(d_i d_j), dx, dy)
+// with (dx, dy) the neighbour's offset in grid units.
+__global__ __launch_bounds__(kMB) void k_mg_features(int64_t n, const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const double *__restrict__ Ae,
+                                                    const double *__restrict__ Av,
+                                                    const double *__restrict__ dsum,
+                                                    const float *__restrict__ logk,
+                                                    float *__restrict__ vfeat,
+                                                    float *__restrict__ efeat) {
+  const int64_t W = n + 1, nv = W * W, i = mg_tid();
+  if (i >= nv) return;
+  const double dmean = dsum[0] / (double)nv;
+  const double di = Av[i];
+  vfeat[2 * i] = (float)log(di / dmean);
+  vfeat[2 * i + 1] = logk[i];
+  const int64_t x = i % W, y = i / W;
+  for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+    const int64_t j = col[e];
+    efeat[3 * (int64_t)e] = (float)(Ae[e] / sqrt(di * Av[j]));
+    efeat[3 * (int64_t)e + 1] = (float)(j % W - x);
+    efeat[3 * (int64_t)e + 2] = (float)(j / W - y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_restrict(int64_t nf, const int32_t *__restrict__ fidx,
+                                                    const T *__restrict__ t, T *__restrict__ rc) {
+  const int64_t nc = nf >> 1, Wc = nc + 1, Wf = nf + 1, c = mg_tid();
+  if (c >= Wc * Wc) return;
+  const int64_t a = c % Wc, b = c / Wc;
+  if (a == 0 || b == 0 || a == nc || b == nc) {
+    rc[c] = T(0);
+    return;
+  }
+  const int64_t p0 = 2 * b * Wf + 2 * a;
+  auto F = [&](int64_t p) -> T { return t[fidx ? fidx[p] : p]; };
+  T acc = F(p0 - 1);
+  acc = acc + F(p0 + 1);
+  acc = acc + F(p0 - Wf);
+  acc = acc + F(p0 + Wf);
+  acc = acc + F(p0 - Wf - 1);
+  acc = acc + F(p0 + Wf + 1);
+  rc[c] = F(p0) + T(0.5) * acc;
+}
+
+// z += P zc
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_prolong(int64_t nf, const int32_t *__restrict__ fpos,
+                                                   const T *__restrict__ zc, T *__restrict__ z) {
+  const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
+  if (i >= nv) return;
+  z[i] = z[i] + prolong_value<T>(fpos ? fpos[i] : i, nf, zc);
+}
+
+// Damped-Jacobi pre-smoothing from z = 0 and the residual it leaves:
+// z = (w r) / d; t = r - A z.
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col,
+                                                      const T *__restrict__ Ae,
+                                                      const T *__restrict__ Av, T omega,
+                                                      const T *__restrict__ r, T *__restrict__ z,
+                                                      T *__restrict__ t) {
+  const int64_t i = mg_tid();
+  if (i >= n) return;
+  const T az = linsys_row<T>(i, rowptr, col, Ae, Av,
+                             [&](int64_t j) { return (omega * r[j]) / Av[j]; });
+  z[i] = (omega * r[i]) / Av[i];
+  t[i] = r[i] - az;
+}
+
+// Coarse-grid correction and damped-Jacobi post-smoothing:
+// y = z + P zc; out = y + (w (r - A y)) / d.
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_t *__restrict__ fpos,
+                                                       const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const T *__restrict__ Ae,
+                                                       const T *__restrict__ Av, T omega,
+                                                       const T *__restrict__ r,
+                                                       const T *__restrict__ z,
+                                                       const T *__restrict__ zc,
+                                                       T *__restrict__ out) {
+  const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
+  if (i >= nv) return;
+  auto y = [&](int64_t j) { return z[j] + prolong_value<T>(fpos ? fpos[j] : j, nf, zc); };
+  const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, y);
+  const T yi = y(i);
+  out[i] = yi + (omega * (r[i] - ay)) / Av[i];
+}
+
+// t = r - A y with y = z (zc null) or y = z + P zc (then also stored).
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_residual(int64_t n, int64_t nf,
+                                                    const int32_t *__restrict__ fpos,
+                                                    const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const T *__restrict__ Ae,
+                                                    const T *__restrict__ Av,
+                                                    const T *__restrict__ r,
+                                                    const T *__restrict__ z,
+                                                    const T *__restrict__ zc, T *__restrict__ y,
+                                                    T *__restrict__ t) {
+  const int64_t i = mg_tid();
+  if (i >= n) return;
+  if (zc) {
+    auto yv = [&](int64_t j) { return z[j] + prolong_value<T>(fpos ? fpos[j] : j, nf, zc); };
+    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, yv);
+    y[i] = yv(i);
+    t[i] = r[i] - ay;
+  } else {
+    const T az = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return z[j]; });
+    t[i] = r[i] - az;
+  }
+}
+
+// Coarsest grid: z = Ainv r (dense, row-major), one wavefront per row.
+template <typename T>
+__global__ __launch_bounds__(kMB) void k_mg_dense(int64_t n, const double *__restrict__ Ainv,
+                                                 const T *__restrict__ r, T *__restrict__ z) {
+  const int64_t row = mg_tid() >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += Ainv[row * n + j] * (double)r[j];
+  s = kr_wave_sum(s);
+  if (lane == 0) z[row] = (T)s;
+}
+
+inline unsigned mg_grid(int64_t n) { return (unsigned)ceil_div(n, kMB); }
+
+} // namespace
+} // namespace esr
+
+using namespace esr;
+
+#define MG_DTYPE_DISPATCH(dtype, CALL)                                         \
+  do {                                                                         \
+    if ((dtype) == ESR_F64) { using T = double; CALL; }                        \
+    else if ((dtype) == ESR_F32) { using T = float; CALL; }                    \
+    else { set_error("multigrid: dtype %d is not float32/float64", (int)(dtype)); return ESR_ERR_UNSUPPORTED; } \
+  } while (0)
+
+ESR_API int esr_mg_pattern(int64_t n, int32_t *rowptr, int32_t *col, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && (n + 1) * (n + 1) * 6 < (int64_t)INT32_MAX, "esr_mg_pattern: n %lld out of range", (long long)n);
+  ESR_REQUIRE(rowptr && col, "esr_mg_pattern: null pointer");
+  const int64_t nv = (n + 1) * (n + 1);
+  k_mg_pattern<<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(n, rowptr, col);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_galerkin(int64_t nf, const int32_t *fidx, const int32_t *fpos,
+                            const int32_t *f_rowptr, const int32_t *f_col, const double *f_Ae,
+                            const double *f_Av, const int32_t *c_rowptr, double *c_Ae,
+                            double *c_Av, int32_t *bad, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_galerkin: fine grid n=%lld is not even", (long long)nf);
+  ESR_REQUIRE((fidx == nullptr) == (fpos == nullptr), "esr_mg_galerkin: give both fidx and fpos or neither");
+  ESR_REQUIRE(f_rowptr && f_col && f_Ae && f_Av && c_rowptr && c_Ae && c_Av && bad,
+              "esr_mg_galerkin: null pointer");
+  const int64_t Wc = nf / 2 + 1;
+  k_mg_galerkin<<<mg_grid(Wc * Wc), kMB, 0, as_stream(stream)>>>(
+      nf, fidx, fpos, f_rowptr, f_col, f_Ae, f_Av, c_rowptr, c_Ae, c_Av, bad);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_features(int64_t n, const int32_t *rowptr, const int32_t *col,
+                            const double *Ae, const double *Av, const double *dsum,
+                            const float *logk, float *vfeat, float *efeat, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_features: n must be >= 1");
+  ESR_REQUIRE(rowptr && col && Ae && Av && dsum && logk && vfeat && efeat,
+              "esr_mg_features: null pointer");
+  const int64_t nv = (n + 1) * (n + 1);
+  k_mg_features<<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(n, rowptr, col, Ae, Av, dsum, logk,
+                                                            vfeat, efeat);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_restrict(int dtype, int64_t nf, const int32_t *fidx, const void *t, void *rc,
+                            void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_restrict: fine grid n=%lld is not even", (long long)nf);
+  ESR_REQUIRE(t && rc, "esr_mg_restrict: null pointer");
+  const int64_t Wc = nf / 2 + 1;
+  MG_DTYPE_DISPATCH(dtype, (k_mg_restrict<T><<<mg_grid(Wc * Wc), kMB, 0, as_stream(stream)>>>(
+                               nf, fidx, (const T *)t, (T *)rc)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_prolong(int dtype, int64_t nf, const int32_t *fpos, const void *zc, void *z,
+                           void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_prolong: fine grid n=%lld is not even", (long long)nf);
+  ESR_REQUIRE(zc && z, "esr_mg_prolong: null pointer");
+  const int64_t nv = (nf + 1) * (nf + 1);
+  MG_DTYPE_DISPATCH(dtype, (k_mg_prolong<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
+                               nf, fpos, (const T *)zc, (T *)z)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr, const int32_t *col,
+                              const void *Ae, const void *Av, double omega, const void *r,
+                              void *z, void *t, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_jacobi_pre: n must be >= 1");
+  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && t, "esr_mg_jacobi_pre: null pointer");
+  MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_pre<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
+                               n, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
+                               (const T *)r, (T *)z, (T *)t)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *fpos, const int32_t *rowptr,
+                               const int32_t *col, const void *Ae, const void *Av, double omega,
+                               const void *r, const void *z, const void *zc, void *out,
+                               void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_jacobi_post: fine grid n=%lld is not even", (long long)nf);
+  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && zc && out,
+              "esr_mg_jacobi_post: null pointer");
+  ESR_REQUIRE(out != z, "esr_mg_jacobi_post: out must not alias z");
+  const int64_t nv = (nf + 1) * (nf + 1);
+  MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
+                               nf, fpos, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
+                               (const T *)r, (const T *)z, (const T *)zc, (T *)out)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *fpos,
+                            const int32_t *rowptr, const int32_t *col, const void *Ae,
+                            const void *Av, const void *r, const void *z, const void *zc,
+                            void *y, void *t, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_residual: n must be >= 1");
+  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && t, "esr_mg_residual: null pointer");
+  if (zc) {
+    ESR_REQUIRE(y && y != z, "esr_mg_residual: y must be given and must not alias z");
+    ESR_REQUIRE(nf >= 2 && nf % 2 == 0 && (nf + 1) * (nf + 1) == n,
+                "esr_mg_residual: n=%lld is not the vertex count of an even grid nf=%lld",
+                (long long)n, (long long)nf);
+  }
+  MG_DTYPE_DISPATCH(dtype, (k_mg_residual<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
+                               n, nf, fpos, rowptr, col, (const T *)Ae, (const T *)Av,
+                               (const T *)r, (const T *)z, (const T *)zc, (T *)y, (T *)t)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_dense(int dtype, int64_t n, const double *Ainv, const void *r, void *z,
+                         void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_dense: n must be >= 1");
+  ESR_REQUIRE(Ainv && r && z && r != z, "esr_mg_dense: null or aliased pointer");
+  MG_DTYPE_DISPATCH(dtype, (k_mg_dense<T><<<(unsigned)ceil_div(n * 64, kMB), kMB, 0,
+                                            as_stream(stream)>>>(n, Ainv, (const T *)r, (T *)z)));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}

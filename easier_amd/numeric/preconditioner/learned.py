@@ -127,6 +127,20 @@ class MeshGraph:
         self.device = dev
 
 
+class GnnBuffers:
+    """Inference buffers of the network on a mesh of n vertices: two
+    (n, 32) bfloat16 activation and message buffers, ||r|| and the norm's
+    reduction workspace."""
+
+    def __init__(self, n, device):
+        self.h = torch.empty((2, n, H), dtype=torch.bfloat16, device=device)
+        self.Q = torch.empty((2, n, H), dtype=torch.bfloat16, device=device)
+        self.rnorm = torch.empty(1, dtype=torch.float64, device=device)
+        probe = torch.empty((n, 1), dtype=torch.float64, device=device)
+        self.norm_ws = torch.empty(ops.aggregate_ws_bytes(probe), dtype=torch.uint8,
+                                   device=device)
+
+
 class LearnedPreconditioner:
     _bindings = 0
 
@@ -189,6 +203,18 @@ class LearnedPreconditioner:
                  self.h.data_ptr(), self.Q.data_ptr(),
                  None if self.a is None else self.a.data_ptr(), 1 if self.train else 0,
                  z.data_ptr(), nat.stream_ptr())
+        return z
+
+    def forward_into(self, g, bufs: 'GnnBuffers', r: torch.Tensor, z: torch.Tensor):
+        """z = M(r) on graph g (any object with n, rowptr, col, efeat, vfeat,
+        d) with the inference buffers bufs -- several meshes at once, e.g. the
+        levels of a multigrid hierarchy. Allocates nothing."""
+        ops.aggregate('norm', r.reshape(-1, 1), out=bufs.rnorm, ws=bufs.norm_ws)
+        nat.call('esr_gnn_forward', self.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
+                 g.efeat.data_ptr(), g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(),
+                 bufs.rnorm.data_ptr(), self.params.data_ptr(), self.packed.data_ptr(),
+                 bufs.h.data_ptr(), bufs.Q.data_ptr(), None, 0, z.data_ptr(),
+                 nat.stream_ptr())
         return z
 
     def backward(self, r: torch.Tensor, dz: torch.Tensor) -> torch.Tensor:

@@ -1,0 +1,251 @@
+"""Multilevel preconditioner z = V(r): a geometric multigrid V(1,1) cycle on
+the 2-triangle mesh, with damped-Jacobi or learned (message-passing network)
+smoothing, executed by the kernels of csrc/k_mg.hip.
+
+No reference counterpart: the reference's preconditioner slot
+(easier/numeric/preconditioner/__init__.py:4) is empty; its solvers take any
+callable M (numeric/solver/cg.py:86, gmres.py:152). A local preconditioner
+(Jacobi, or a network that sees 3 mesh hops) leaves the O(n^1/2) Krylov
+iteration count of the Poisson problem unchanged; the coarse grids carry the
+smooth error components across the mesh in one cycle.
+
+Hierarchy (set-up, on the device): level 0 is the caller's operator in its
+own vertex order, mapped to grid positions through the vertex coordinates;
+level l+1 has half the cells per side, lexicographic order, and the Galerkin
+operator P^T A P (P = P1 interpolation between the nested meshes); the
+coarsening stops at n <= coarsest cells per side (or an odd n), where the
+cycle solves with the dense inverse.
+
+Cycle, per level (r in, z out):
+  Jacobi:  z0 = w r/d; t = r - A z0; zc = V(P^T t);
+           y = z0 + P zc; z = y + w (r - A y)/d           (3 kernels)
+  learned: z0 = S(r); t = r - A z0; zc = V(P^T t);
+           y = z0 + P zc; t = r - A y; z = y + S(t)
+with S the network (z = (r + s net(r/s, features))/d, s = rms(r)). The
+learned cycle is not linear in r (S is positively homogeneous only), which
+is why it runs inside flexible CG.
+"""
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ...core import ops
+
+_c_i32, _c_i64, _c_vp, _c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
+nat.register_signatures({
+    'esr_mg_pattern': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_galerkin': (_c_i32, [_c_i64] + [_c_vp] * 11),
+    'esr_mg_features': (_c_i32, [_c_i64] + [_c_vp] * 9),
+    'esr_mg_restrict': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_prolong': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_jacobi_pre': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl,
+                                   _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_jacobi_post': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                    _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_residual': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 11),
+    'esr_mg_dense': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+})
+
+
+def grid_positions(points, n):
+    """Lexicographic grid position J (n+1) + I of each vertex (I = rint(x n),
+    J = rint(y n)): exact for the 2-triangle mesh on the unit square, with or
+    without interior jitter below half a cell. Raises ValueError unless the
+    vertices are the (n+1)^2 grid points, each once."""
+    pts = np.asarray(points, np.float64)
+    W = n + 1
+    if pts.ndim != 2 or pts.shape[0] != W * W:
+        raise ValueError(f'{pts.shape[0]} vertices are not the vertices of a {n} x {n} grid')
+    I = np.rint(pts[:, 0] * n).astype(np.int64)
+    J = np.rint(pts[:, 1] * n).astype(np.int64)
+    if I.min() < 0 or J.min() < 0 or I.max() > n or J.max() > n:
+        raise ValueError('vertex coordinates outside the unit square')
+    pos = J * W + I
+    if np.bincount(pos, minlength=W * W).max() != 1:
+        raise ValueError('two vertices map to one grid position')
+    return pos
+
+
+def levels_for(n, coarsest=16):
+    """Grid sizes of the hierarchy: n, n/2, ... down to n <= coarsest or odd."""
+    out = [int(n)]
+    while out[-1] % 2 == 0 and out[-1] > coarsest:
+        out.append(out[-1] // 2)
+    return out
+
+
+class _Graph:
+    """The network's static inputs on a coarse level (inference only)."""
+
+    def __init__(self, n, rowptr, col, efeat, vfeat, d):
+        self.n, self.rowptr, self.col, self.efeat, self.vfeat, self.d = \
+            n, rowptr, col, efeat, vfeat, d
+
+
+class Level:
+    def __init__(self, n_grid, rowptr, col, Ae, Av, fpos=None, fidx=None):
+        self.n_grid = n_grid
+        self.nv = (n_grid + 1) ** 2
+        self.rowptr, self.col, self.Ae, self.Av = rowptr, col, Ae, Av
+        self.fpos, self.fidx = fpos, fidx
+        self.Ainv = None
+        self.graph = None        # network inputs (learned smoothing)
+        self.bufs = None
+
+
+class MultigridPreconditioner:
+    """V(1,1)-cycle preconditioner. smoother: None (damped Jacobi, weight
+    omega) or a LearnedPreconditioner (its network smooths every level with at
+    least `learned_min_grid` cells per side; coarser levels use Jacobi).
+    setup() builds the hierarchy for one operator; apply(r, z) runs one cycle
+    and allocates nothing, so it can be captured in a HIP graph."""
+
+    _bindings = 0
+    fcg_chunk = 4        # flexible-CG steps per captured graph
+
+    def __init__(self, smoother=None, omega=0.8, coarsest=16, learned_min_grid=64):
+        self.smoother = smoother
+        self.omega = float(omega)
+        self.coarsest = int(coarsest)
+        self.learned_min_grid = int(learned_min_grid)
+        self.levels: List[Level] = []
+        self.binding = 0
+
+    # -- set-up ---------------------------------------------------------------
+    def setup(self, rowptr, col, Ae, Av, n_grid, fpos=None, logk=None, graph0=None):
+        """rowptr/col int32, Ae/Av float64 device tensors of the fine operator
+        (Linsys form: off-diagonal CSR + diagonal) on the n_grid x n_grid
+        2-triangle mesh; fpos (host int array, vertex -> grid position; None =
+        lexicographic). With a learned smoother: logk (host, per vertex in the
+        operator's order) and graph0, the network's inputs on the fine level
+        (a MeshGraph)."""
+        dev = Av.device
+        nv = Av.shape[0]
+        if nv != (n_grid + 1) ** 2:
+            raise ValueError(f'operator of {nv} rows is not on a {n_grid} x {n_grid} grid')
+        sizes = levels_for(n_grid, self.coarsest)
+        if sizes[-1] > 32:
+            raise ValueError(f'grid {n_grid} coarsens to an odd grid of {sizes[-1]} cells per '
+                             'side: the coarsest grid must have at most 32')
+        i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        fp = fi = None
+        if fpos is not None:
+            fpos = np.asarray(fpos, np.int64)
+            if not np.array_equal(fpos, np.arange(nv)):
+                fidx = np.empty_like(fpos)
+                fidx[fpos] = np.arange(nv)
+                fp, fi = i32(fpos), i32(fidx)
+        self.levels = [Level(n_grid, rowptr, col, Ae, Av, fp, fi)]
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        sp = nat.stream_ptr()
+        for nc in sizes[1:]:
+            f = self.levels[-1]
+            W = nc + 1
+            c_rowptr = torch.empty(W * W + 1, dtype=torch.int32, device=dev)
+            c_col = torch.empty((W - 1) * (6 * W - 2), dtype=torch.int32, device=dev)
+            nat.call('esr_mg_pattern', nc, c_rowptr.data_ptr(), c_col.data_ptr(), sp)
+            c_Ae = torch.empty(c_col.shape[0], dtype=torch.float64, device=dev)
+            c_Av = torch.empty(W * W, dtype=torch.float64, device=dev)
+            P = lambda t: None if t is None else t.data_ptr()
+            nat.call('esr_mg_galerkin', f.n_grid, P(f.fidx), P(f.fpos), f.rowptr.data_ptr(),
+                     f.col.data_ptr(), f.Ae.data_ptr(), f.Av.data_ptr(), c_rowptr.data_ptr(),
+                     c_Ae.data_ptr(), c_Av.data_ptr(), bad.data_ptr(), sp)
+            self.levels.append(Level(nc, c_rowptr, c_col, c_Ae, c_Av))
+        if int(bad.item()):
+            raise ValueError('the operator couples vertices that are not neighbours on the '
+                             '2-triangle mesh: geometric coarsening does not apply')
+        last = self.levels[-1]
+        rp, cl = last.rowptr.cpu().numpy(), last.col.cpu().numpy()
+        dense = np.zeros((last.nv, last.nv))
+        rows = np.repeat(np.arange(last.nv), np.diff(rp))
+        np.add.at(dense, (rows, cl), last.Ae.cpu().numpy())
+        dense[np.arange(last.nv), np.arange(last.nv)] += last.Av.cpu().numpy()
+        last.Ainv = torch.from_numpy(np.linalg.inv(dense)).to(dev)
+        for k, L in enumerate(self.levels):
+            L.z = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+            L.t = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+            if k > 0:
+                L.r = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+                L.zo = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+        if self.smoother is not None:
+            self._setup_learned(logk, graph0, fpos)
+        MultigridPreconditioner._bindings += 1
+        self.binding = MultigridPreconditioner._bindings
+        return self
+
+    def _setup_learned(self, logk, graph0, fpos):
+        from .learned import GnnBuffers
+        if logk is None or graph0 is None:
+            raise ValueError('a learned smoother needs logk and the fine-level MeshGraph')
+        dev = self.levels[0].Av.device
+        n0 = self.levels[0].n_grid
+        W0 = n0 + 1
+        lk = np.asarray(logk, np.float64).reshape(-1)
+        lex = np.empty(W0 * W0)
+        lex[np.arange(W0 * W0) if fpos is None else np.asarray(fpos)] = lk
+        lex = lex.reshape(W0, W0)
+        dsum = torch.empty(1, dtype=torch.float64, device=dev)
+        for k, L in enumerate(self.levels):
+            if L.Ainv is not None or L.n_grid < self.learned_min_grid:
+                continue
+            if k == 0:
+                L.graph = graph0
+            else:
+                step = 2 ** k
+                lkl = torch.from_numpy(np.ascontiguousarray(lex[::step, ::step],
+                                                            np.float32).reshape(-1)).to(dev)
+                vfeat = torch.empty((L.nv, 2), dtype=torch.float32, device=dev)
+                efeat = torch.empty((L.col.shape[0], 3), dtype=torch.float32, device=dev)
+                ops.aggregate('sum', L.Av.reshape(-1, 1), out=dsum)
+                nat.call('esr_mg_features', L.n_grid, L.rowptr.data_ptr(), L.col.data_ptr(),
+                         L.Ae.data_ptr(), L.Av.data_ptr(), dsum.data_ptr(), lkl.data_ptr(),
+                         vfeat.data_ptr(), efeat.data_ptr(), nat.stream_ptr())
+                L.graph = _Graph(L.nv, L.rowptr, L.col, efeat, vfeat, L.Av)
+            L.bufs = GnnBuffers(L.nv, dev)
+            L.y = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+            L.s = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+
+    # -- cycle ----------------------------------------------------------------
+    def apply(self, r: torch.Tensor, z: torch.Tensor):
+        if not self.levels:
+            raise RuntimeError('MultigridPreconditioner.setup() first')
+        self._cycle(0, r, z)
+        return z
+
+    def __call__(self, r):
+        return self.apply(r, torch.empty_like(r))
+
+    def _cycle(self, k, r, out):
+        L = self.levels[k]
+        sp = nat.stream_ptr()
+        F64 = nat.F64
+        if L.Ainv is not None:
+            nat.call('esr_mg_dense', F64, L.nv, L.Ainv.data_ptr(), r.data_ptr(),
+                     out.data_ptr(), sp)
+            return
+        C = self.levels[k + 1]
+        P = lambda t: None if t is None else t.data_ptr()
+        csr = (L.rowptr.data_ptr(), L.col.data_ptr(), L.Ae.data_ptr(), L.Av.data_ptr())
+        if L.graph is None:
+            nat.call('esr_mg_jacobi_pre', F64, L.nv, *csr, self.omega, r.data_ptr(),
+                     L.z.data_ptr(), L.t.data_ptr(), sp)
+            nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(),
+                     C.r.data_ptr(), sp)
+            self._cycle(k + 1, C.r, C.zo)
+            nat.call('esr_mg_jacobi_post', F64, L.n_grid, P(L.fpos), *csr, self.omega,
+                     r.data_ptr(), L.z.data_ptr(), C.zo.data_ptr(), out.data_ptr(), sp)
+            return
+        S = self.smoother
+        S.forward_into(L.graph, L.bufs, r, L.z)
+        nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
+                 L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
+        nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
+                 sp)
+        self._cycle(k + 1, C.r, C.zo)
+        nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
+                 L.z.data_ptr(), C.zo.data_ptr(), L.y.data_ptr(), L.t.data_ptr(), sp)
+        S.forward_into(L.graph, L.bufs, L.t, L.s)
+        ops.ewise('add', L.y, L.s, out=out)

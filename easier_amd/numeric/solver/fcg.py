@@ -210,21 +210,34 @@ class FlexibleCG(esr.Module):
         if why or self.x.dtype != torch.float64:
             raise RuntimeError(f'FlexibleCG needs a float64 Linsys ({why})')
         op = LinsysCSR(self.A)
+        # steps per captured chunk: a strong preconditioner converges in few
+        # steps, and steps after convergence still run its cycle
         self.easier_kernels = FcgKernels(op.rowptr, op.col, op.Ae, op.Av, op.n,
-                                         self.x.device)
+                                         self.x.device,
+                                         chunk=getattr(self.easier_M, 'fcg_chunk', 16))
         if self.easier_M is not None:
+            from ..preconditioner.multigrid import MultigridPreconditioner, grid_positions
+            M = self.easier_M
             f = self.easier_features
             if f is None:
-                raise RuntimeError('a learned preconditioner needs mesh_features')
+                raise RuntimeError('a learned or multigrid preconditioner needs mesh_features')
             o = self.x.easier_order
             o = np.arange(op.n) if o is None else o.numpy()
-            rowptr = self.A.reducer.easier_host_rowptr
-            row = np.repeat(np.arange(op.n), np.diff(rowptr))
-            graph = MeshGraph(row, self.A.selector.easier_host_idx,
-                              op.Av.cpu().numpy(), op.Ae.cpu().numpy(),
-                              np.asarray(f['points'])[o], np.asarray(f['logk']).reshape(-1)[o],
-                              f['n_grid'], self.x.device)
-            self.easier_M.bind(graph)
+            points = np.asarray(f['points'])[o]
+            logk = None if f.get('logk') is None else np.asarray(f['logk']).reshape(-1)[o]
+            learned = M.smoother if isinstance(M, MultigridPreconditioner) else M
+            graph = None
+            if learned is not None:
+                rowptr = self.A.reducer.easier_host_rowptr
+                row = np.repeat(np.arange(op.n), np.diff(rowptr))
+                graph = MeshGraph(row, self.A.selector.easier_host_idx,
+                                  op.Av.cpu().numpy(), op.Ae.cpu().numpy(), points, logk,
+                                  f['n_grid'], self.x.device)
+            if isinstance(M, MultigridPreconditioner):
+                M.setup(op.rowptr, op.col, op.Ae, op.Av, f['n_grid'],
+                        grid_positions(points, f['n_grid']), logk, graph)
+            else:
+                M.bind(graph)
 
     def solve(self, rtol=1e-8, atol=None, maxiter=None, window=200,
               debug_iter=None) -> Dict[str, Any]:

@@ -198,9 +198,10 @@ def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
 
 
 def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every=200,
-          clip=1.0, loss='energy'):
+          clip=1.0, loss='energy', log=None):
     """data: list of (Problem, pairs). Returns [(step, loss)] every
-    `log_every` steps (and the mean loss of each epoch as (step, mean))."""
+    `log_every` steps (and the mean loss of each epoch as (step, mean));
+    log(step, running mean) is called at the same steps."""
     rng = np.random.default_rng(seed)
     losses = []
     step = 0
@@ -210,7 +211,7 @@ def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every
         order.sort(key=lambda t: t[0])           # one bind per problem per epoch
         bound = None
         tot = 0.0
-        for i, j in order:
+        for k, (i, j) in enumerate(order, 1):
             prob, pairs = data[i]
             if bound != i:
                 pre.bind(prob.graph, train=True)
@@ -225,6 +226,8 @@ def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every
             tot += l
             if step % log_every == 0 or step == 1:
                 losses.append((step, l))
+                if log is not None:
+                    log(step, tot / k)
         losses.append((step, tot / max(len(order), 1)))
     return losses
 
@@ -243,14 +246,33 @@ def mean_loss(pre: Optional[LearnedPreconditioner], data, loss='energy'):
     return tot / max(cnt, 1)
 
 
-def evaluate_compiled(pre: LearnedPreconditioner, n, seed, rtol=1e-8, maxiter=None,
-                      reps=2):
-    """The drop-in comparison: the variable-coefficient problem of `seed` on
-    the n x n mesh, compiled with esr.compile(..., 'hip') twice -- Jacobi-PCG
-    (the fused CG solver) and flexible CG with the learned preconditioner --
-    both from x = 0 to `rtol` relative residual. Times are the best of `reps`
-    solves after one warm-up; the true float64 residual ||b - A x|| of both
-    final iterates is recomputed on the host."""
+METHODS = ('jacobi', 'learned', 'mg', 'mg-learned')
+
+
+def make_preconditioner(method, pre: Optional[LearnedPreconditioner], **mg_args):
+    """The preconditioner of flexible CG for an evaluation method: 'learned'
+    = the network alone, 'mg' = multigrid V-cycle with damped-Jacobi
+    smoothing, 'mg-learned' = the same cycle with the network as smoother."""
+    from ..numeric.preconditioner.multigrid import MultigridPreconditioner
+    if method == 'learned':
+        return pre
+    if method == 'mg':
+        return MultigridPreconditioner(None, **mg_args)
+    if method == 'mg-learned':
+        return MultigridPreconditioner(pre, **mg_args)
+    raise ValueError(f'unknown method {method!r}')
+
+
+def evaluate_compiled(pre: Optional[LearnedPreconditioner], n, seed, rtol=1e-8, maxiter=None,
+                      reps=2, methods=('jacobi', 'learned'), mg_args=None):
+    """The drop-in comparison on the variable-coefficient problem of `seed`
+    on the n x n mesh: each method compiled with esr.compile(..., 'hip') --
+    'jacobi' is the fused Jacobi-PCG (CG solver), the others flexible CG with
+    make_preconditioner(method) -- all from x = 0 to `rtol` relative
+    residual. Solve times are the best of `reps` solves after one warm-up;
+    `setup_seconds` is the compile (renumbering, device transfer and, for
+    multigrid, the hierarchy). The true float64 residual ||b - A x|| of each
+    final iterate is recomputed on the host."""
     import scipy.sparse as sp
     from .. import compile as esr_compile
     from ..numeric.solver import CG, FlexibleCG
@@ -275,25 +297,36 @@ def evaluate_compiled(pre: LearnedPreconditioner, n, seed, rtol=1e-8, maxiter=No
                 best = dt if best is None else min(best, dt)
         return best, info
 
-    eqn = wp.PoissonP1(sys_)
-    cg = CG(eqn.A, eqn.b, eqn.x, eqn.M)
-    esr_compile([eqn, cg], 'hip')
-    t, info = timed(lambda: cg.solve(rtol=rtol, maxiter=maxiter), eqn.x)
-    xj = eqn.x.collect().numpy()
-    out['jacobi'] = dict(seconds=t, iters=info['iters'],
-                         true_rel_residual=float(np.linalg.norm(sys_['b'] - A @ xj)) / bnorm)
-    del eqn, cg
-    eqn = wp.PoissonP1(sys_)
-    fcg = FlexibleCG(eqn.A, eqn.b, eqn.x, pre,
-                     dict(points=mesh['points'], logk=logk, n_grid=n))
-    esr_compile([eqn, fcg], 'hip')
-    t, info = timed(lambda: fcg.solve(rtol=rtol, maxiter=maxiter), eqn.x)
-    xl = eqn.x.collect().numpy()
-    out['learned'] = dict(seconds=t, iters=info['iters'], fallback_at=info['fallback_at'],
-                          true_rel_residual=float(np.linalg.norm(sys_['b'] - A @ xl)) / bnorm)
-    out['speedup'] = out['jacobi']['seconds'] / out['learned']['seconds']
-    out['iteration_ratio'] = out['jacobi']['iters'] / max(out['learned']['iters'], 1)
-    out['rel_diff_x'] = float(np.linalg.norm(xl - xj) / np.linalg.norm(xj))
+    xs = {}
+    for method in methods:
+        eqn = wp.PoissonP1(sys_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if method == 'jacobi':
+            sol = CG(eqn.A, eqn.b, eqn.x, eqn.M)
+        else:
+            M = make_preconditioner(method, pre, **(mg_args or {}))
+            sol = FlexibleCG(eqn.A, eqn.b, eqn.x, M,
+                             dict(points=mesh['points'], logk=logk, n_grid=n))
+        esr_compile([eqn, sol], 'hip')
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        t, info = timed(lambda: sol.solve(rtol=rtol, maxiter=maxiter), eqn.x)
+        x = eqn.x.collect().numpy()
+        xs[method] = x
+        out[method] = dict(seconds=t, iters=info['iters'], setup_seconds=setup,
+                           fallback_at=info.get('fallback_at'),
+                           true_rel_residual=float(np.linalg.norm(sys_['b'] - A @ x)) / bnorm)
+        _log(f'{method}: {out[method]}')
+        del eqn, sol
+    if 'jacobi' in out:
+        for method in methods:
+            if method != 'jacobi':
+                r = out[method]
+                r['speedup_vs_jacobi'] = out['jacobi']['seconds'] / r['seconds']
+                r['iteration_ratio'] = out['jacobi']['iters'] / max(r['iters'], 1)
+                r['rel_diff_x'] = float(np.linalg.norm(xs[method] - xs['jacobi']) /
+                                        np.linalg.norm(xs['jacobi']))
     return out
 
 
@@ -322,11 +355,13 @@ def main(argv=None):
     tr.add_argument('--policy-rhs', type=int, default=8)
     tr.add_argument('--out', required=True)
     ev = sub.add_parser('evaluate', help='learned FCG vs Jacobi-PCG on unseen seeds')
-    ev.add_argument('--weights', required=True)
+    ev.add_argument('--weights', required=True,
+                    help="an .npz from 'train', or 'untrained' (the network equal to Jacobi)")
     ev.add_argument('--n', type=int, default=2048)
     ev.add_argument('--seeds', type=int, nargs='+', default=[1])
     ev.add_argument('--rtol', type=float, default=1e-8)
     ev.add_argument('--maxiter', type=int, default=50000)
+    ev.add_argument('--methods', nargs='+', choices=METHODS, default=list(METHODS))
     args = ap.parse_args(argv)
     import json
     if args.cmd == 'train':
@@ -345,8 +380,9 @@ def main(argv=None):
         def epochs(tag):
             for ep in range(args.epochs):
                 lr = args.lr * (0.5 ** ep)
-                losses = train(pre, data, epochs=1, lr=lr, seed=ep, log_every=1000,
-                               loss=args.loss)
+                losses = train(pre, data, epochs=1, lr=lr, seed=ep, log_every=500,
+                               loss=args.loss,
+                               log=lambda st, m: _log(f'{tag} step {st}: running mean {m:.4f}'))
                 _log(f'{tag} epoch {ep + 1}/{args.epochs} lr {lr:.2e}: mean '
                      f'{losses[-1][1]:.4f} ({time.perf_counter() - t0:.1f} s)')
         epochs('pcg pairs')
@@ -367,9 +403,11 @@ def main(argv=None):
                 seeds=np.arange(args.seed0, args.seed0 + args.problems))
         _log(f'saved {args.out}')
     else:
-        pre = lp.load(args.weights)
+        pre = lp.load(args.weights) if args.weights != 'untrained' else \
+            LearnedPreconditioner(3, lp.init_params(3, 0))
         for s in args.seeds:
-            res = evaluate_compiled(pre, args.n, s, args.rtol, maxiter=args.maxiter)
+            res = evaluate_compiled(pre, args.n, s, args.rtol, maxiter=args.maxiter,
+                                    methods=args.methods)
             print(json.dumps(res), flush=True)
 
 

@@ -394,6 +394,53 @@ int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
 int esr_fcg_set(void *state, double tol2, int64_t maxit, void *stream);
 int esr_fcg_read(const void *state, double *scalars, int64_t *ints, void *stream);
 
+/* ---- geometric multigrid on the 2-triangle mesh (new; a preconditioner for
+ * the reference's callable-M slot, numeric/solver/cg.py:86 and
+ * easier/numeric/preconditioner/__init__.py:4) ------------------------------
+ * Grids of n cells per side, (n+1)^2 vertices; coarse levels lexicographic
+ * (vertex (I, J) at J (n+1) + I), the finest level in any order given by
+ * fpos (index -> position) and fidx (position -> index), both NULL for
+ * lexicographic. P = P1 interpolation between nested meshes without the
+ * coarse boundary columns; restriction = P^T.
+ * esr_mg_pattern: vertex graph of the n-grid (rowptr (n+1)^2+1, col).
+ * esr_mg_galerkin: coarse Ae/Av = P^T A P on that pattern (boundary rows
+ *   identity); *bad counts fine couplings outside the coarse stencil (0 for
+ *   an operator on the 2-triangle mesh's graph).
+ * esr_mg_features: the learned smoother's inputs on a lexicographic level.
+ * esr_mg_restrict: rc = P^T t. esr_mg_prolong: z += P zc.
+ * esr_mg_jacobi_pre: z = w r / d, t = r - A z.
+ * esr_mg_jacobi_post: y = z + P zc, out = y + w (r - A y) / d.
+ * esr_mg_residual: t = r - A y, y = z (zc NULL) or y = z + P zc (stored).
+ * esr_mg_dense: z = Ainv r (coarsest grid, Ainv row-major float64).
+ * dtype ESR_F64 or ESR_F32 (vectors and the operator's Ae/Av).           */
+int esr_mg_pattern(int64_t n, int32_t *rowptr, int32_t *col, void *stream);
+int esr_mg_galerkin(int64_t nf, const int32_t *fidx, const int32_t *fpos,
+                    const int32_t *f_rowptr, const int32_t *f_col,
+                    const double *f_Ae, const double *f_Av,
+                    const int32_t *c_rowptr, double *c_Ae, double *c_Av,
+                    int32_t *bad, void *stream);
+int esr_mg_features(int64_t n, const int32_t *rowptr, const int32_t *col,
+                    const double *Ae, const double *Av, const double *dsum,
+                    const float *logk, float *vfeat, float *efeat, void *stream);
+int esr_mg_restrict(int dtype, int64_t nf, const int32_t *fidx, const void *t,
+                    void *rc, void *stream);
+int esr_mg_prolong(int dtype, int64_t nf, const int32_t *fpos, const void *zc,
+                   void *z, void *stream);
+int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr,
+                      const int32_t *col, const void *Ae, const void *Av,
+                      double omega, const void *r, void *z, void *t, void *stream);
+int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *fpos,
+                       const int32_t *rowptr, const int32_t *col,
+                       const void *Ae, const void *Av, double omega,
+                       const void *r, const void *z, const void *zc, void *out,
+                       void *stream);
+int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *fpos,
+                    const int32_t *rowptr, const int32_t *col, const void *Ae,
+                    const void *Av, const void *r, const void *z, const void *zc,
+                    void *y, void *t, void *stream);
+int esr_mg_dense(int dtype, int64_t n, const double *Ainv, const void *r, void *z,
+                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@ import easier_amd.numeric.solver.mixed  # noqa: F401
 import easier_amd.numeric.solver.batched  # noqa: F401
 import easier_amd.numeric.solver.fcg  # noqa: F401
 import easier_amd.numeric.preconditioner.learned  # noqa: F401
+import easier_amd.numeric.preconditioner.multigrid  # noqa: F401
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
