@@ -303,13 +303,18 @@ def mixed_config4(reps=1):
     return out
 
 
-def learned_config5(weights, seeds=(1,), reps=1):
+def learned_config5(weights, seeds=(1,), reps=1,
+                    methods=('jacobi', 'mg', 'mg-learned')):
     """Learned-preconditioner flexible CG against Jacobi-PCG on unseen
-    variable-coefficient problems on the 2048 x 2048 mesh, both to 1e-8."""
+    variable-coefficient problems on the 2048 x 2048 mesh, all to 1e-8:
+    'mg-learned' is flexible CG preconditioned by a multigrid V-cycle whose
+    smoother is the trained network, 'mg' the same cycle with damped-Jacobi
+    smoothing (the method without the network)."""
     from easier_amd.numeric.preconditioner import learned as lp
     from easier_amd.workloads import learned as wl
     pre = lp.load(weights)
-    return [wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000)
+    return [wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000,
+                                 methods=methods)
             for s in seeds]
 
 
@@ -354,7 +359,7 @@ def main():
                     help='comma list of extra workloads measured after the headline: '
                          '3 (batched PCG), 4 (mixed precision), 5 (learned FCG)')
     ap.add_argument('--weights', default=os.path.join(REPO, 'easier_amd', 'workloads',
-                                                      'gnn_poisson_l3_h32.npz'))
+                                                      'gnn_smoother_l3_h32.npz'))
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy a float64 array of MIB MiB 5 times (the known-size '
                          'kernel that scales the HBM byte counters)')

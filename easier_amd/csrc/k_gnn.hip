@@ -23,7 +23,10 @@ namespace esr {
 namespace {
 
 constexpr int H = 32;
-constexpr int kB = 256;
+constexpr int kB = 256;   // backward / elementwise kernels
+// forward kernels: one wavefront per workgroup, so the LDS round trips of
+// the MFMA layout changes (c_to_rows) synchronise one wave, not four
+constexpr int kFB = 64;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
@@ -96,7 +99,7 @@ __device__ __forceinline__ Lane lane_of(int64_t n) {
   const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   Lane ln;
-  ln.v = (blk * (kB / 64) + w) * 16 + (l & 15);
+  ln.v = (blk * (kFB / 64) + w) * 16 + (l & 15);
   ln.g = l >> 4;
   ln.ok = ln.v < n;
   return ln;
@@ -105,13 +108,13 @@ __device__ __forceinline__ Lane lane_of(int64_t n) {
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kB) void k_gnn_encode(int64_t n, const double *__restrict__ r,
+__global__ __launch_bounds__(kFB) void k_gnn_encode(int64_t n, const double *__restrict__ r,
                                                   const float *__restrict__ vfeat,
                                                   const double *__restrict__ rnorm,
                                                   const float *__restrict__ prm,
                                                   const __bf16 *__restrict__ WqT0,
                                                   __bf16 *__restrict__ h0, __bf16 *__restrict__ Q0) {
-  __shared__ float T[kB / 64][16][33];
+  __shared__ float T[kFB / 64][16][33];
   const Lane ln = lane_of(n);
   const double s = rnorm[0] / sqrt((double)n);
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(kB) void k_gnn_encode(int64_t n, const double *__re
 }
 
 template <bool LAST>
-__global__ __launch_bounds__(kB) void k_gnn_layer(
+__global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gnn_layer(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const float *__restrict__ efeat, const __bf16 *__restrict__ h_in,
     const __bf16 *__restrict__ Q_in, const float *__restrict__ lw,
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kB) void k_gnn_layer(
     __bf16 *__restrict__ h_out, __bf16 *__restrict__ Q_out, float *__restrict__ a_save,
     const float *__restrict__ wout, const double *__restrict__ r,
     const double *__restrict__ d, const double *__restrict__ rnorm, double *__restrict__ z) {
-  __shared__ float T[kB / 64][16][33];
+  __shared__ float T[kFB / 64][16][33];
   float(*Tw)[33] = T[threadIdx.x >> 6];
   const Lane ln = lane_of(n);
   const int g = ln.g;
@@ -172,16 +175,39 @@ __global__ __launch_bounds__(kB) void k_gnn_layer(
   }
   if (ln.ok) {
     const int32_t e0 = rowptr[ln.v], e1 = rowptr[ln.v + 1];
-    for (int32_t e = e0; e < e1; ++e) {
-      const int64_t j = col[e];
-      const bf16x8 qj = *reinterpret_cast<const bf16x8 *>(Q_in + j * H + 8 * g);
-      const float f0 = efeat[3 * (int64_t)e], f1 = efeat[3 * (int64_t)e + 1],
-                  f2 = efeat[3 * (int64_t)e + 2];
+    auto message = [&](const bf16x8 &qj, float f0, float f1, float f2) {
 #pragma unroll
       for (int f = 0; f < 8; ++f) {
         const float ew = (f0 * we0[f] + f1 * we1[f]) + f2 * we2[f];
         const float pre = ((P[f] + (float)qj[f]) + ew) + bm[f];
         acc[f] = acc[f] + relu(pre);
+      }
+    };
+    if (e1 > e0) {
+      // the first kSlots edges: positions clamped to the row, so every load
+      // is valid and all are in flight together; folded in edge order
+      constexpr int kSlots = 6;
+      int32_t es[kSlots], js[kSlots];
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) es[k] = e0 + k < e1 ? e0 + k : e1 - 1;
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) js[k] = col[es[k]];
+      bf16x8 qs[kSlots];
+      float fs[kSlots][3];
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        qs[k] = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)js[k] * H + 8 * g);
+        fs[k][0] = efeat[3 * (int64_t)es[k]];
+        fs[k][1] = efeat[3 * (int64_t)es[k] + 1];
+        fs[k][2] = efeat[3 * (int64_t)es[k] + 2];
+      }
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k)
+        if (e0 + k < e1) message(qs[k], fs[k][0], fs[k][1], fs[k][2]);
+      for (int32_t e = e0 + kSlots; e < e1; ++e) {
+        const int64_t j = col[e];
+        message(*reinterpret_cast<const bf16x8 *>(Q_in + j * H + 8 * g), efeat[3 * (int64_t)e],
+                efeat[3 * (int64_t)e + 1], efeat[3 * (int64_t)e + 2]);
       }
     }
     if (a_save) {
@@ -436,7 +462,7 @@ __global__ __launch_bounds__(kB) void k_adam(int64_t np_, float *__restrict__ p,
 
 using namespace esr;
 
-static unsigned vgrid(int64_t n) { return (unsigned)ceil_div(n, 64); }      // 64 vertices / block
+static unsigned vgrid(int64_t n) { return (unsigned)ceil_div(n, 16 * (kFB / 64)); } // 16 vertices / wave
 static unsigned tgrid(int64_t n) { return (unsigned)ceil_div(n, kB); }
 
 ESR_API int esr_gnn_sizes(int L, int64_t *n_params, int64_t *n_packed) {
@@ -475,18 +501,18 @@ ESR_API int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr, const int32
   __bf16 *hb = (__bf16 *)h, *qb = (__bf16 *)Q;
   auto hbuf = [&](int l) { return hb + (int64_t)(save ? l : (l & 1)) * n * H; };
   auto qbuf = [&](int l) { return qb + (int64_t)(save ? l : (l & 1)) * n * H; };
-  k_gnn_encode<<<vgrid(n), kB, 0, s>>>(n, r, vfeat, rnorm, params, pk + kPWq, hbuf(0), qbuf(0));
+  k_gnn_encode<<<vgrid(n), kFB, 0, s>>>(n, r, vfeat, rnorm, params, pk + kPWq, hbuf(0), qbuf(0));
   ESR_LAUNCHED();
   for (int l = 0; l < L; ++l) {
     const float *lw = params + o.layer(l);
     const __bf16 *lp = pk + (int64_t)l * kPLayer;
     float *asv = save ? a + (int64_t)l * n * H : nullptr;
     if (l + 1 < L) {
-      k_gnn_layer<false><<<vgrid(n), kB, 0, s>>>(
+      k_gnn_layer<false><<<vgrid(n), kFB, 0, s>>>(
           n, rowptr, col, efeat, hbuf(l), qbuf(l), lw, lp, pk + (int64_t)(l + 1) * kPLayer + kPWq,
           hbuf(l + 1), qbuf(l + 1), asv, nullptr, nullptr, nullptr, nullptr, nullptr);
     } else {
-      k_gnn_layer<true><<<vgrid(n), kB, 0, s>>>(
+      k_gnn_layer<true><<<vgrid(n), kFB, 0, s>>>(
           n, rowptr, col, efeat, hbuf(l), qbuf(l), lw, lp, nullptr, save ? hbuf(l + 1) : nullptr,
           nullptr, asv, params + o.w_out(), r, d, rnorm, z);
     }

@@ -137,6 +137,21 @@ def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=
     return out
 
 
+def generate_pairs_random(prob: Problem, count, seed, store_dtype=torch.float32):
+    """(r, e, ||e||^2, r.e) with e uniform[-1, 1] on the interior vertices
+    (zero on the boundary, seeded) and r = A e: errors with every frequency,
+    the part a smoother has to remove."""
+    gen = torch.Generator().manual_seed(seed)
+    inner = torch.from_numpy(~prob.sys['boundary'])
+    out = []
+    for _ in range(count):
+        e = (torch.rand(prob.n, generator=gen, dtype=torch.float64) * 2 - 1) * inner
+        ed = e.to(prob.device)
+        r = spmv(prob, ed)
+        out.append((r, ed.to(store_dtype), _dot(ed, ed), _dot(r, ed)))
+    return out
+
+
 def generate_pairs_fcg(prob: Problem, pre: LearnedPreconditioner, n_rhs, seed,
                        every=32, max_snaps=24, store_dtype=torch.float32):
     """Pairs from the iterates of flexible CG preconditioned by the network
@@ -178,8 +193,65 @@ def generate_pairs_fcg(prob: Problem, pre: LearnedPreconditioner, n_rhs, seed,
     return out
 
 
+class TwoGrid:
+    """The two-grid cycle a smoother is trained for: on a Problem's
+    multigrid hierarchy, the error left after smoothing with z and the exact
+    coarse-grid correction is Q (e - z), Q = I - P Ac^-1 P^T A (the
+    A-orthogonal projector onto the complement of the coarse space). Ac^-1 is
+    applied as `coarse_cycles` stationary V-cycles (damped-Jacobi smoothing)
+    from the coarse level down."""
+
+    def __init__(self, prob: Problem, coarse_cycles=4, omega=0.8):
+        from ..numeric.preconditioner.multigrid import MultigridPreconditioner
+        coarsest = min(16, max(1, prob.n_grid // 2))
+        self.mg = MultigridPreconditioner(None, omega, coarsest=coarsest).setup(
+            prob.rowptr, prob.col, prob.Ae, prob.Av, prob.n_grid)
+        if len(self.mg.levels) < 2:
+            raise ValueError(f'a {prob.n_grid} x {prob.n_grid} mesh has no coarse level')
+        self.cycles = coarse_cycles
+        dev = prob.device
+        n, nc = self.mg.levels[0].nv, self.mg.levels[1].nv
+        mk = lambda m: torch.zeros(m, dtype=torch.float64, device=dev)
+        self.t, self.y, self.aqw, self.zero = mk(n), mk(n), mk(n), mk(n)
+        self.rc, self.ec, self.res, self.de = mk(nc), mk(nc), mk(nc), mk(nc)
+
+    @staticmethod
+    def _csr(L):
+        return (L.rowptr.data_ptr(), L.col.data_ptr(), L.Ae.data_ptr(), L.Av.data_ptr())
+
+    def coarse_solve(self):
+        """ec ~ Ac^-1 rc."""
+        L1 = self.mg.levels[1]
+        sp = nat.stream_ptr()
+        self.mg._cycle(1, self.rc, self.ec)
+        for _ in range(self.cycles - 1):
+            nat.call('esr_mg_residual', nat.F64, L1.nv, L1.n_grid, None, *self._csr(L1),
+                     self.rc.data_ptr(), self.ec.data_ptr(), None, None, self.res.data_ptr(), sp)
+            self.mg._cycle(1, self.res, self.de)
+            ops.ewise('add', self.ec, self.de, out=self.ec)
+
+    def aq(self, r, z):
+        """A Q (e - z) for A e = r (into self.aqw)."""
+        L0 = self.mg.levels[0]
+        sp = nat.stream_ptr()
+        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
+                 r.data_ptr(), z.data_ptr(), None, None, self.t.data_ptr(), sp)
+        nat.call('esr_mg_restrict', nat.F64, L0.n_grid, None, self.t.data_ptr(),
+                 self.rc.data_ptr(), sp)
+        self.coarse_solve()
+        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
+                 self.t.data_ptr(), self.zero.data_ptr(), self.ec.data_ptr(),
+                 self.y.data_ptr(), self.aqw.data_ptr(), sp)
+        return self.aqw
+
+
 def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
     """(loss, dloss/dz) of the preconditioned vector z = M(r).
+
+    kind 'twogrid': ||Q (e - z)||_A^2 / ||e||_A^2, the A-norm error left by a
+        two-grid cycle that pre-smooths with z (TwoGrid; prob.twogrid); its
+        gradient is -2 A Q (e - z) / ||e||_A^2 (Q is A-orthogonal). Not
+        scale-invariant: the smoother is used with unit step.
 
     kind 'error':  ||z - e||^2 / ||e||^2.
     kind 'energy': 1 - (z.r)|z.r| / ((z.Az) (r.e)) -- for z.r > 0 the
@@ -190,6 +262,12 @@ def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
     if kind == 'error':
         diff = ops.ewise('sub', z, e)
         return _dot(diff, diff) / en2, ops.ewise('mul', diff, 2.0 / en2)
+    if kind == 'twogrid':
+        if getattr(prob, 'twogrid', None) is None:
+            prob.twogrid = TwoGrid(prob)
+        aqw = prob.twogrid.aq(r, z)
+        w = ops.ewise('sub', e.to(torch.float64), z)
+        return _dot(w, aqw) / re, ops.ewise('mul', aqw, -2.0 / re)
     q = spmv(prob, z)
     zr, zq = _dot(z, r), _dot(z, q)
     loss = 1.0 - zr * abs(zr) / (zq * re)
@@ -232,15 +310,16 @@ def train(pre: LearnedPreconditioner, data, epochs=1, lr=1e-3, seed=0, log_every
     return losses
 
 
-def mean_loss(pre: Optional[LearnedPreconditioner], data, loss='energy'):
+def mean_loss(pre: Optional[LearnedPreconditioner], data, loss='energy', omega=1.0):
     """Mean pair_loss over the pairs; pre None = the Jacobi preconditioner
-    z = r / d (the untrained network)."""
+    z = omega r / d (omega = 1: the untrained network)."""
     tot, cnt = 0.0, 0
     for prob, pairs in data:
         if pre is not None:
             pre.bind(prob.graph)
         for r, e, en2, re in pairs:
-            z = ops.ewise('div', r, prob.Av) if pre is None else pre(r)
+            z = ops.ewise('mul', ops.ewise('div', r, prob.Av), omega) if pre is None \
+                else pre(r)
             tot += pair_loss(prob, r, e, en2, re, z, loss)[0]
             cnt += 1
     return tot / max(cnt, 1)
@@ -342,17 +421,20 @@ def main(argv=None):
     sub = ap.add_subparsers(dest='cmd', required=True)
     tr = sub.add_parser('train', help='generate (r, e) pairs with Jacobi-PCG on seeded '
                         'problems and train the network on the GPU')
-    tr.add_argument('--n', type=int, default=512)
+    tr.add_argument('--n', type=int, nargs='+', default=[512],
+                    help='mesh sizes; problems are dealt over them in turn')
     tr.add_argument('--problems', type=int, default=24)
     tr.add_argument('--rhs', type=int, default=36)
     tr.add_argument('--epochs', type=int, default=4)
     tr.add_argument('--lr', type=float, default=1e-3)
     tr.add_argument('--layers', type=int, default=3)
     tr.add_argument('--seed0', type=int, default=1000)
-    tr.add_argument('--loss', choices=['energy', 'error'], default='energy')
-    tr.add_argument('--policy-rounds', type=int, default=1,
+    tr.add_argument('--loss', choices=['energy', 'error', 'twogrid'], default='twogrid')
+    tr.add_argument('--policy-rounds', type=int, default=0,
                     help='rounds of pairs from flexible CG with the network itself')
     tr.add_argument('--policy-rhs', type=int, default=8)
+    tr.add_argument('--random-pairs', type=int, default=0,
+                    help='pairs per problem with a uniform random error e and r = A e')
     tr.add_argument('--out', required=True)
     ev = sub.add_parser('evaluate', help='learned FCG vs Jacobi-PCG on unseen seeds')
     ev.add_argument('--weights', required=True,
@@ -362,19 +444,27 @@ def main(argv=None):
     ev.add_argument('--rtol', type=float, default=1e-8)
     ev.add_argument('--maxiter', type=int, default=50000)
     ev.add_argument('--methods', nargs='+', choices=METHODS, default=list(METHODS))
+    ev.add_argument('--learned-min-grid', type=int, default=64,
+                    help='multigrid levels with at least this many cells per side use '
+                         'the network as smoother')
     args = ap.parse_args(argv)
     import json
     if args.cmd == 'train':
         data, t0 = [], time.perf_counter()
+        sizes = [args.n[k % len(args.n)] for k in range(args.problems)]
         for k in range(args.problems):
-            prob = Problem(args.n, args.seed0 + k)
+            prob = Problem(sizes[k], args.seed0 + k)
             pairs = generate_pairs(prob, args.rhs, seed=args.seed0 + k)
+            if args.random_pairs:
+                pairs += generate_pairs_random(prob, args.random_pairs,
+                                               seed=2 * 10 ** 6 + args.seed0 + k)
             data.append((prob, pairs))
-            _log(f'problem {k + 1}/{args.problems} (seed {args.seed0 + k}): {len(pairs)} '
-                 f'pairs, {time.perf_counter() - t0:.1f} s')
+            _log(f'problem {k + 1}/{args.problems} ({sizes[k]}^2, seed {args.seed0 + k}): '
+                 f'{len(pairs)} pairs, {time.perf_counter() - t0:.1f} s')
         npairs = sum(len(p) for _, p in data)
         base = mean_loss(None, data, args.loss)
-        _log(f'{npairs} pairs; Jacobi {args.loss} loss {base:.4f}')
+        base08 = mean_loss(None, data, args.loss, omega=0.8)
+        _log(f'{npairs} pairs; Jacobi {args.loss} loss {base:.4f} (damped 0.8: {base08:.4f})')
         pre = LearnedPreconditioner(args.layers, lp.init_params(args.layers, args.seed0))
 
         def epochs(tag):
@@ -396,10 +486,10 @@ def main(argv=None):
                  f'({time.perf_counter() - t0:.1f} s)')
             epochs(f'round {rnd + 1}')
         final = mean_loss(pre, data, args.loss)
-        base = mean_loss(None, data, args.loss)
-        _log(f'trained {args.loss} loss {final:.4f} (Jacobi {base:.4f})')
-        lp.save(args.out, pre, n_train=args.n, pairs=npairs, jacobi_loss=base,
-                trained_loss=final, loss=args.loss,
+        _log(f'trained {args.loss} loss {final:.4f} (Jacobi {base:.4f}, damped 0.8 '
+             f'{base08:.4f})')
+        lp.save(args.out, pre, n_train=np.asarray(sizes), pairs=npairs, jacobi_loss=base,
+                damped_jacobi_loss=base08, trained_loss=final, loss=args.loss,
                 seeds=np.arange(args.seed0, args.seed0 + args.problems))
         _log(f'saved {args.out}')
     else:
@@ -407,7 +497,8 @@ def main(argv=None):
             LearnedPreconditioner(3, lp.init_params(3, 0))
         for s in args.seeds:
             res = evaluate_compiled(pre, args.n, s, args.rtol, maxiter=args.maxiter,
-                                    methods=args.methods)
+                                    methods=args.methods,
+                                    mg_args=dict(learned_min_grid=args.learned_min_grid))
             print(json.dumps(res), flush=True)
 
 
