@@ -20,10 +20,13 @@ Cycle, per level (r in, z out):
   Jacobi:  z0 = w r/d; t = r - A z0; zc = V(P^T t);
            y = z0 + P zc; z = y + w (r - A y)/d           (3 kernels)
   learned: z0 = S(r); t = r - A z0; zc = V(P^T t);
+           then the Jacobi post-smoothing above (default), or
            y = z0 + P zc; t = r - A y; z = y + S(t)
 with S the network (z = (r + s net(r/s, features))/d, s = rms(r)). The
 learned cycle is not linear in r (S is positively homogeneous only), which
-is why it runs inside flexible CG.
+is why it runs inside flexible CG. The network smooths in one position and
+damped Jacobi in the other: on the 2048 x 2048 variable-coefficient problem
+that took 10 flexible-CG steps, the network in both positions 15.
 """
 import ctypes
 from typing import List, Optional
@@ -99,15 +102,19 @@ class Level:
 class MultigridPreconditioner:
     """V(1,1)-cycle preconditioner. smoother: None (damped Jacobi, weight
     omega) or a LearnedPreconditioner (its network smooths every level with at
-    least `learned_min_grid` cells per side; coarser levels use Jacobi).
+    least `learned_min_grid` cells per side, before the coarse-grid correction
+    if learned_pre and after it if learned_post, damped Jacobi otherwise;
+    coarser levels use Jacobi).
     setup() builds the hierarchy for one operator; apply(r, z) runs one cycle
     and allocates nothing, so it can be captured in a HIP graph."""
 
     _bindings = 0
     fcg_chunk = 4        # flexible-CG steps per captured graph
 
-    def __init__(self, smoother=None, omega=0.8, coarsest=16, learned_min_grid=64):
+    def __init__(self, smoother=None, omega=0.8, coarsest=16, learned_min_grid=64,
+                 learned_pre=True, learned_post=False):
         self.smoother = smoother
+        self.learned_pre, self.learned_post = bool(learned_pre), bool(learned_post)
         self.omega = float(omega)
         self.coarsest = int(coarsest)
         self.learned_min_grid = int(learned_min_grid)
@@ -239,13 +246,21 @@ class MultigridPreconditioner:
                      r.data_ptr(), L.z.data_ptr(), C.zo.data_ptr(), out.data_ptr(), sp)
             return
         S = self.smoother
-        S.forward_into(L.graph, L.bufs, r, L.z)
-        nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
-                 L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
+        if self.learned_pre:
+            S.forward_into(L.graph, L.bufs, r, L.z)
+            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
+                     L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
+        else:
+            nat.call('esr_mg_jacobi_pre', F64, L.nv, *csr, self.omega, r.data_ptr(),
+                     L.z.data_ptr(), L.t.data_ptr(), sp)
         nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
                  sp)
         self._cycle(k + 1, C.r, C.zo)
-        nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
-                 L.z.data_ptr(), C.zo.data_ptr(), L.y.data_ptr(), L.t.data_ptr(), sp)
-        S.forward_into(L.graph, L.bufs, L.t, L.s)
-        ops.ewise('add', L.y, L.s, out=out)
+        if self.learned_post:
+            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
+                     L.z.data_ptr(), C.zo.data_ptr(), L.y.data_ptr(), L.t.data_ptr(), sp)
+            S.forward_into(L.graph, L.bufs, L.t, L.s)
+            ops.ewise('add', L.y, L.s, out=out)
+        else:
+            nat.call('esr_mg_jacobi_post', F64, L.n_grid, P(L.fpos), *csr, self.omega,
+                     r.data_ptr(), L.z.data_ptr(), C.zo.data_ptr(), out.data_ptr(), sp)

@@ -106,9 +106,12 @@ def spmv(prob: Problem, x: torch.Tensor) -> torch.Tensor:
     return ops.ewise('add', off, ops.ewise('mul', x, prob.Av))
 
 
-def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=torch.float32):
+def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=torch.float64):
     """(r_k, e_k, ||e_k||^2, r_k.e_k) from Jacobi-PCG runs with uniform
-    random right-hand sides (seeded); r_k.e_k = ||e_k||_A^2 since A e_k = r_k."""
+    random right-hand sides (seeded); r_k.e_k = ||e_k||_A^2 since A e_k = r_k.
+    The iterates are kept in float64: e_k = x* - x_k of a late snapshot is
+    far smaller than x_k, and float32 rounding of x_k would put
+    high-frequency noise into e_k that A e_k amplifies."""
     pcg = _Pcg(prob)
     out = []
     gen = torch.Generator().manual_seed(seed)
@@ -125,7 +128,7 @@ def generate_pairs(prob: Problem, n_rhs, seed, snapshots=SNAPSHOTS, store_dtype=
             st = pcg.read()
             if st['done']:
                 break
-            snaps.append((pcg.vec['r'].clone(), pcg.vec['x'].to(store_dtype)))
+            snaps.append((pcg.vec['r'].clone(), pcg.vec['x'].to(store_dtype, copy=True)))
         while not pcg.read()['done']:
             pcg.steps(256)
         xstar = pcg.vec['x']
@@ -173,14 +176,14 @@ def generate_pairs_fcg(prob: Problem, pre: LearnedPreconditioner, n_rhs, seed,
         fk.op(RESIDUAL_OP, b=bd, x=x)
         tol = 1e-10 * float(np.sqrt(fk.read()['bb']))
         nat.call('esr_fcg_set', fk.state.data_ptr(), tol * tol, -1, nat.stream_ptr())
-        snaps = [(fk.r.clone(), x.to(store_dtype))]
+        snaps = [(fk.r.clone(), x.to(store_dtype, copy=True))]
         fk._step(pre, x, True)
         while len(snaps) < max_snaps:
             fk._chunk_graph(pre, x).replay()
             s = fk.read()
             if s['done']:
                 break
-            snaps.append((fk.r.clone(), x.to(store_dtype)))
+            snaps.append((fk.r.clone(), x.to(store_dtype, copy=True)))
         pcg.init(bd, 1e-10, maxiter=max(20000, 4 * prob.n_grid * prob.n_grid))
         while not pcg.read()['done']:
             pcg.steps(256)
@@ -230,19 +233,24 @@ class TwoGrid:
             self.mg._cycle(1, self.res, self.de)
             ops.ewise('add', self.ec, self.de, out=self.ec)
 
-    def aq(self, r, z):
-        """A Q (e - z) for A e = r (into self.aqw)."""
+    def project(self, t):
+        """A Q v for t = A v (into self.aqw; self.y = P Ac^-1 P^T t)."""
         L0 = self.mg.levels[0]
         sp = nat.stream_ptr()
-        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
-                 r.data_ptr(), z.data_ptr(), None, None, self.t.data_ptr(), sp)
-        nat.call('esr_mg_restrict', nat.F64, L0.n_grid, None, self.t.data_ptr(),
+        nat.call('esr_mg_restrict', nat.F64, L0.n_grid, None, t.data_ptr(),
                  self.rc.data_ptr(), sp)
         self.coarse_solve()
         nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
-                 self.t.data_ptr(), self.zero.data_ptr(), self.ec.data_ptr(),
+                 t.data_ptr(), self.zero.data_ptr(), self.ec.data_ptr(),
                  self.y.data_ptr(), self.aqw.data_ptr(), sp)
         return self.aqw
+
+    def aq(self, r, z):
+        """A Q (e - z) for A e = r (into self.aqw)."""
+        L0 = self.mg.levels[0]
+        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
+                 r.data_ptr(), z.data_ptr(), None, None, self.t.data_ptr(), nat.stream_ptr())
+        return self.project(self.t)
 
 
 def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
@@ -252,6 +260,10 @@ def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
         two-grid cycle that pre-smooths with z (TwoGrid; prob.twogrid); its
         gradient is -2 A Q (e - z) / ||e||_A^2 (Q is A-orthogonal). Not
         scale-invariant: the smoother is used with unit step.
+    kind 'twogrid_post': the same cycle followed by damped-Jacobi
+        post-smoothing S = I - w D^-1 A (w = 0.8), as the multigrid
+        preconditioner runs it: ||S Q (e - z)||_A^2 / ||e||_A^2, gradient
+        -2 A Q S^2 Q (e - z) / ||e||_A^2 (S and Q are A-self-adjoint).
 
     kind 'error':  ||z - e||^2 / ||e||^2.
     kind 'energy': 1 - (z.r)|z.r| / ((z.Az) (r.e)) -- for z.r > 0 the
@@ -262,12 +274,22 @@ def pair_loss(prob: Problem, r, e, en2, re, z, kind='energy'):
     if kind == 'error':
         diff = ops.ewise('sub', z, e)
         return _dot(diff, diff) / en2, ops.ewise('mul', diff, 2.0 / en2)
-    if kind == 'twogrid':
+    if kind in ('twogrid', 'twogrid_post'):
         if getattr(prob, 'twogrid', None) is None:
             prob.twogrid = TwoGrid(prob)
-        aqw = prob.twogrid.aq(r, z)
+        tg = prob.twogrid
+        aqw = tg.aq(r, z)
         w = ops.ewise('sub', e.to(torch.float64), z)
-        return _dot(w, aqw) / re, ops.ewise('mul', aqw, -2.0 / re)
+        if kind == 'twogrid':
+            return _dot(w, aqw) / re, ops.ewise('mul', aqw, -2.0 / re)
+        omega = 0.8
+        u = ops.ewise('sub', w, tg.y)                                  # Q w
+        s1 = ops.ewise('sub', u, ops.ewise('mul', ops.ewise('div', aqw, prob.Av), omega))
+        as1 = spmv(prob, s1)
+        loss = _dot(s1, as1) / re
+        s2 = ops.ewise('sub', s1, ops.ewise('mul', ops.ewise('div', as1, prob.Av), omega))
+        aq2 = tg.project(spmv(prob, s2))
+        return loss, ops.ewise('mul', aq2, -2.0 / re)
     q = spmv(prob, z)
     zr, zq = _dot(z, r), _dot(z, q)
     loss = 1.0 - zr * abs(zr) / (zq * re)
@@ -429,7 +451,8 @@ def main(argv=None):
     tr.add_argument('--lr', type=float, default=1e-3)
     tr.add_argument('--layers', type=int, default=3)
     tr.add_argument('--seed0', type=int, default=1000)
-    tr.add_argument('--loss', choices=['energy', 'error', 'twogrid'], default='twogrid')
+    tr.add_argument('--loss', choices=['energy', 'error', 'twogrid', 'twogrid_post'],
+                    default='twogrid_post')
     tr.add_argument('--policy-rounds', type=int, default=0,
                     help='rounds of pairs from flexible CG with the network itself')
     tr.add_argument('--policy-rhs', type=int, default=8)
@@ -447,6 +470,8 @@ def main(argv=None):
     ev.add_argument('--learned-min-grid', type=int, default=64,
                     help='multigrid levels with at least this many cells per side use '
                          'the network as smoother')
+    ev.add_argument('--learned-smoothing', choices=['pre', 'post', 'both'], default='pre',
+                    help='where the network smooths on those levels (damped Jacobi elsewhere)')
     args = ap.parse_args(argv)
     import json
     if args.cmd == 'train':
@@ -498,7 +523,9 @@ def main(argv=None):
         for s in args.seeds:
             res = evaluate_compiled(pre, args.n, s, args.rtol, maxiter=args.maxiter,
                                     methods=args.methods,
-                                    mg_args=dict(learned_min_grid=args.learned_min_grid))
+                                    mg_args=dict(learned_min_grid=args.learned_min_grid,
+                                                 learned_pre=args.learned_smoothing != 'post',
+                                                 learned_post=args.learned_smoothing != 'pre'))
             print(json.dumps(res), flush=True)
 
 

@@ -203,14 +203,16 @@ def hierarchy(A, n, fpos=None, coarsest=16):
     return levels
 
 
-def vcycle(levels, r, omega=0.8, l=0):
-    """z ~ A^{-1} r by one V(1,1) cycle: z = w D^-1 r; t = r - A z;
-    z += P V(P^T t); z += w D^-1 (r - A z)."""
+def vcycle(levels, r, omega=0.8, l=0, omega_pre=None):
+    """z ~ A^{-1} r by one V(1,1) cycle: z = w' D^-1 r; t = r - A z;
+    z += P V(P^T t); z += w D^-1 (r - A z), w' = omega_pre (default w) on
+    every level."""
     L = levels[l]
     if L.Ainv is not None:
         return L.Ainv @ r
-    z = omega * r / L.d
+    wp = omega if omega_pre is None else omega_pre
+    z = wp * r / L.d
     t = r - L.A @ z
-    zc = vcycle(levels, L.P.T @ t, omega, l + 1)
+    zc = vcycle(levels, L.P.T @ t, omega, l + 1, omega_pre)
     z = z + L.P @ zc
     return z + omega * (r - L.A @ z) / L.d

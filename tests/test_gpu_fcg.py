@@ -66,7 +66,7 @@ def test_random_network_converges_and_fallback_path(gpu):
     assert _true_residual(sys_, eqn.x.collect()) <= 1.01e-8 * bn
 
 
-@pytest.mark.parametrize('kind', ['energy', 'error', 'twogrid'])
+@pytest.mark.parametrize('kind', ['energy', 'error', 'twogrid', 'twogrid_post'])
 def test_pair_loss_gradient_matches_finite_differences(gpu, kind):
     prob = wl.Problem(12, 3)
     r, e, en2, re = wl.generate_pairs(prob, n_rhs=1, seed=0)[3]
@@ -82,7 +82,7 @@ def test_pair_loss_gradient_matches_finite_differences(gpu, kind):
     lm_ = wl.pair_loss(prob, r, e, en2, re, z - h * v, kind)[0]
     fd = (lp_ - lm_) / (2 * h)
     assert float((dz * v).sum()) == pytest.approx(fd, rel=1e-5, abs=1e-12)
-    if kind == 'twogrid':
+    if kind.startswith('twogrid'):
         # smoothing with the exact error leaves nothing; no smoothing leaves
         # the part of e outside the coarse space, a fraction in (0, 1)
         ed = e.double()

@@ -136,13 +136,20 @@ def test_untrained_learned_smoother_is_undamped_jacobi(gpu):
     from easier_amd.numeric.preconditioner.learned import MeshGraph
     graph = MeshGraph(s['row'], s['col'], s['Av'], s['Ae'], pts, logk, n, gpu)
     S = LearnedPreconditioner(3, gnn.init_params(3, 0))
-    M = mg.MultigridPreconditioner(S, coarsest=4, learned_min_grid=4)
-    M.setup(*csr, n, None, logk, graph)
-    assert all(L.graph is not None for L in M.levels[:-1])
     ref = omg.hierarchy(A, n, None, coarsest=4)
     r = np.random.default_rng(3).standard_normal(fpos.size)
+    # the network before and after the coarse-grid correction
+    M = mg.MultigridPreconditioner(S, coarsest=4, learned_min_grid=4, learned_post=True)
+    M.setup(*csr, n, None, logk, graph)
+    assert all(L.graph is not None for L in M.levels[:-1])
     z = M(torch.from_numpy(r).to(gpu)).cpu().numpy()
     want = omg.vcycle(ref, r, 1.0)
+    np.testing.assert_allclose(z, want, rtol=0, atol=1e-12 * abs(want).max())
+    # the default: network before, damped Jacobi (w = 0.8) after
+    M = mg.MultigridPreconditioner(S, coarsest=4, learned_min_grid=4)
+    M.setup(*csr, n, None, logk, graph)
+    z = M(torch.from_numpy(r).to(gpu)).cpu().numpy()
+    want = omg.vcycle(ref, r, 0.8, omega_pre=1.0)
     np.testing.assert_allclose(z, want, rtol=0, atol=1e-12 * abs(want).max())
 
 
