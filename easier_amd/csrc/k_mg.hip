@@ -8,7 +8,8 @@
 // (squares split along the SW-NE diagonal); vertex (I, J) sits at
 // lexicographic position J (n+1) + I. Coarse levels are stored in
 // lexicographic order; the finest level may be in any order, given by
-// fpos (fine index -> position) and fidx (position -> fine index).
+// fpos (fine index -> position) and fidx (position -> fine index); the cycle
+// kernels read a parent map built from fpos instead (esr_mg_parents).
 //
 // Transfer. P is P1 interpolation between the nested meshes: a fine vertex
 // at an even-even position copies its coarse parent, one on a horizontal,
@@ -31,19 +32,34 @@ constexpr int kMB = 256;
 
 __device__ __forceinline__ int64_t mg_tid() { return (int64_t)blockIdx.x * kMB + threadIdx.x; }
 
-// (P zc) at fine position p of an nf-grid; coarse boundary values read as 0.
-template <typename T>
-__device__ __forceinline__ T prolong_value(int64_t p, int64_t nf, const T *__restrict__ zc) {
-  const int64_t Wf = nf + 1, nc = nf >> 1, Wc = nc + 1;
-  const int64_t I = p % Wf, J = p / Wf;
-  auto Z = [&](int64_t a, int64_t b) -> T {
-    return (a > 0 && b > 0 && a < nc && b < nc) ? zc[b * Wc + a] : T(0);
+// Parent map of a fine level: par[2i], par[2i+1] = the coarse indices of
+// vertex i's parents, -1 for a coarse boundary vertex (zero value), and
+// par[2i+1] = -2 when i sits on a coarse vertex (one parent, weight 1; else
+// two parents, weight 1/2 each). Built once per hierarchy, so the cycle
+// kernels do no index division.
+__device__ __forceinline__ void parents_of(int64_t p, int64_t nf, int32_t &p0, int32_t &p1) {
+  const uint32_t Wf = (uint32_t)(nf + 1), nc = (uint32_t)(nf >> 1), Wc = nc + 1;
+  const uint32_t I = (uint32_t)p % Wf, J = (uint32_t)p / Wf;
+  const uint32_t a = I >> 1, b = J >> 1;
+  auto C = [&](uint32_t x, uint32_t y) -> int32_t {
+    return (x > 0 && y > 0 && x < nc && y < nc) ? (int32_t)(y * Wc + x) : -1;
   };
-  const int64_t a = I >> 1, b = J >> 1;
-  if (!(I & 1) && !(J & 1)) return Z(a, b);
-  if (!(J & 1)) return T(0.5) * (Z(a, b) + Z(a + 1, b));
-  if (!(I & 1)) return T(0.5) * (Z(a, b) + Z(a, b + 1));
-  return T(0.5) * (Z(a, b) + Z(a + 1, b + 1));
+  p0 = C(a, b);
+  if (!(I & 1) && !(J & 1)) p1 = -2;
+  else if (!(J & 1)) p1 = C(a + 1, b);
+  else if (!(I & 1)) p1 = C(a, b + 1);
+  else p1 = C(a + 1, b + 1);
+}
+
+// (P zc) at fine vertex i: the parent's value, or 0.5 (first + second).
+template <typename T>
+__device__ __forceinline__ T prolong_value(int64_t i, const int32_t *__restrict__ par,
+                                           const T *__restrict__ zc) {
+  const int2 pp = reinterpret_cast<const int2 *>(par)[i];
+  const T v0 = pp.x >= 0 ? zc[pp.x] : T(0);
+  if (pp.y == -2) return v0;
+  const T v1 = pp.y >= 0 ? zc[pp.y] : T(0);
+  return T(0.5) * (v0 + v1);
 }
 
 // ---------------------------------------------------------------------------
@@ -69,6 +85,15 @@ __global__ __launch_bounds__(kMB) void k_mg_pattern(int64_t n, int32_t *__restri
     const int64_t xx = x + dx[s], yy = y + dy[s];
     if (xx >= 0 && xx < W && yy >= 0 && yy < W) col[k++] = (int32_t)(yy * W + xx);
   }
+}
+
+__global__ __launch_bounds__(kMB) void k_mg_parents(int64_t nf, const int32_t *__restrict__ fpos,
+                                                   int32_t *__restrict__ par) {
+  const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
+  if (i >= nv) return;
+  int32_t p0, p1;
+  parents_of(fpos ? fpos[i] : i, nf, p0, p1);
+  reinterpret_cast<int2 *>(par)[i] = make_int2(p0, p1);
 }
 
 // Slot of coarse offset (dx, dy): 0 = diagonal, 1..6 = SW, S, W, E, N, NE;
@@ -171,7 +196,7 @@ __global__ __launch_bounds__(kMB) void k_mg_restrict(int64_t nf, const int32_t *
                                                     const T *__restrict__ t, T *__restrict__ rc) {
   const int64_t nc = nf >> 1, Wc = nc + 1, Wf = nf + 1, c = mg_tid();
   if (c >= Wc * Wc) return;
-  const int64_t a = c % Wc, b = c / Wc;
+  const int64_t a = (uint32_t)c % (uint32_t)Wc, b = (uint32_t)c / (uint32_t)Wc;
   if (a == 0 || b == 0 || a == nc || b == nc) {
     rc[c] = T(0);
     return;
@@ -189,11 +214,11 @@ __global__ __launch_bounds__(kMB) void k_mg_restrict(int64_t nf, const int32_t *
 
 // z += P zc
 template <typename T>
-__global__ __launch_bounds__(kMB) void k_mg_prolong(int64_t nf, const int32_t *__restrict__ fpos,
+__global__ __launch_bounds__(kMB) void k_mg_prolong(int64_t nf, const int32_t *__restrict__ par,
                                                    const T *__restrict__ zc, T *__restrict__ z) {
   const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
   if (i >= nv) return;
-  z[i] = z[i] + prolong_value<T>(fpos ? fpos[i] : i, nf, zc);
+  z[i] = z[i] + prolong_value<T>(i, par, zc);
 }
 
 // Damped-Jacobi pre-smoothing from z = 0 and the residual it leaves:
@@ -214,9 +239,12 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const int32_t 
 }
 
 // Coarse-grid correction and damped-Jacobi post-smoothing:
-// y = z + P zc; out = y + (w (r - A y)) / d.
+// y = z + P zc; out = y + (w (r - A y)) / d. With zc null, y = z (the
+// correction already added by k_mg_prolong): the fused form evaluates P at
+// every neighbour (a rowptr -> col -> parent -> zc load chain), which runs
+// about 3x slower on the finest level than the two-pass form.
 template <typename T>
-__global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_t *__restrict__ fpos,
+__global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_t *__restrict__ par,
                                                        const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const T *__restrict__ Ae,
@@ -227,16 +255,21 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_
                                                        T *__restrict__ out) {
   const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
   if (i >= nv) return;
-  auto y = [&](int64_t j) { return z[j] + prolong_value<T>(fpos ? fpos[j] : j, nf, zc); };
-  const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, y);
-  const T yi = y(i);
-  out[i] = yi + (omega * (r[i] - ay)) / Av[i];
+  if (zc) {
+    auto y = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
+    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, y);
+    const T yi = y(i);
+    out[i] = yi + (omega * (r[i] - ay)) / Av[i];
+  } else {
+    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return z[j]; });
+    out[i] = z[i] + (omega * (r[i] - ay)) / Av[i];
+  }
 }
 
 // t = r - A y with y = z (zc null) or y = z + P zc (then also stored).
 template <typename T>
 __global__ __launch_bounds__(kMB) void k_mg_residual(int64_t n, int64_t nf,
-                                                    const int32_t *__restrict__ fpos,
+                                                    const int32_t *__restrict__ par,
                                                     const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ col,
                                                     const T *__restrict__ Ae,
@@ -248,7 +281,7 @@ __global__ __launch_bounds__(kMB) void k_mg_residual(int64_t n, int64_t nf,
   const int64_t i = mg_tid();
   if (i >= n) return;
   if (zc) {
-    auto yv = [&](int64_t j) { return z[j] + prolong_value<T>(fpos ? fpos[j] : j, nf, zc); };
+    auto yv = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
     const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, yv);
     y[i] = yv(i);
     t[i] = r[i] - ay;
@@ -341,14 +374,25 @@ ESR_API int esr_mg_restrict(int dtype, int64_t nf, const int32_t *fidx, const vo
   ESR_GUARD_END
 }
 
-ESR_API int esr_mg_prolong(int dtype, int64_t nf, const int32_t *fpos, const void *zc, void *z,
+ESR_API int esr_mg_parents(int64_t nf, const int32_t *fpos, int32_t *par, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_parents: fine grid n=%lld is not even", (long long)nf);
+  ESR_REQUIRE(par, "esr_mg_parents: null pointer");
+  const int64_t nv = (nf + 1) * (nf + 1);
+  k_mg_parents<<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(nf, fpos, par);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_prolong(int dtype, int64_t nf, const int32_t *par, const void *zc, void *z,
                            void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_prolong: fine grid n=%lld is not even", (long long)nf);
-  ESR_REQUIRE(zc && z, "esr_mg_prolong: null pointer");
+  ESR_REQUIRE(par && zc && z, "esr_mg_prolong: null pointer");
   const int64_t nv = (nf + 1) * (nf + 1);
   MG_DTYPE_DISPATCH(dtype, (k_mg_prolong<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
-                               nf, fpos, (const T *)zc, (T *)z)));
+                               nf, par, (const T *)zc, (T *)z)));
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
@@ -368,25 +412,25 @@ ESR_API int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr, const
   ESR_GUARD_END
 }
 
-ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *fpos, const int32_t *rowptr,
+ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *par, const int32_t *rowptr,
                                const int32_t *col, const void *Ae, const void *Av, double omega,
                                const void *r, const void *z, const void *zc, void *out,
                                void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_jacobi_post: fine grid n=%lld is not even", (long long)nf);
-  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && zc && out,
+  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && out && (par || !zc),
               "esr_mg_jacobi_post: null pointer");
   ESR_REQUIRE(out != z, "esr_mg_jacobi_post: out must not alias z");
   const int64_t nv = (nf + 1) * (nf + 1);
   MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
-                               nf, fpos, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
+                               nf, par, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
                                (const T *)r, (const T *)z, (const T *)zc, (T *)out)));
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
 }
 
-ESR_API int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *fpos,
+ESR_API int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *par,
                             const int32_t *rowptr, const int32_t *col, const void *Ae,
                             const void *Av, const void *r, const void *z, const void *zc,
                             void *y, void *t, void *stream) {
@@ -394,13 +438,14 @@ ESR_API int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *fpo
   ESR_REQUIRE(n >= 1, "esr_mg_residual: n must be >= 1");
   ESR_REQUIRE(rowptr && col && Ae && Av && r && z && t, "esr_mg_residual: null pointer");
   if (zc) {
-    ESR_REQUIRE(y && y != z, "esr_mg_residual: y must be given and must not alias z");
+    ESR_REQUIRE(par && y && y != z,
+                "esr_mg_residual: with zc, par and y must be given and y must not alias z");
     ESR_REQUIRE(nf >= 2 && nf % 2 == 0 && (nf + 1) * (nf + 1) == n,
                 "esr_mg_residual: n=%lld is not the vertex count of an even grid nf=%lld",
                 (long long)n, (long long)nf);
   }
   MG_DTYPE_DISPATCH(dtype, (k_mg_residual<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
-                               n, nf, fpos, rowptr, col, (const T *)Ae, (const T *)Av,
+                               n, nf, par, rowptr, col, (const T *)Ae, (const T *)Av,
                                (const T *)r, (const T *)z, (const T *)zc, (T *)y, (T *)t)));
   ESR_LAUNCHED();
   return ESR_OK;

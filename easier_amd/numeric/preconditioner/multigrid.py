@@ -18,7 +18,7 @@ cycle solves with the dense inverse.
 
 Cycle, per level (r in, z out):
   Jacobi:  z0 = w r/d; t = r - A z0; zc = V(P^T t);
-           y = z0 + P zc; z = y + w (r - A y)/d           (3 kernels)
+           y = z0 + P zc; z = y + w (r - A y)/d           (4 kernels)
   learned: z0 = S(r); t = r - A z0; zc = V(P^T t);
            then the Jacobi post-smoothing above (default), or
            y = z0 + P zc; t = r - A y; z = y + S(t)
@@ -42,6 +42,7 @@ nat.register_signatures({
     'esr_mg_pattern': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
     'esr_mg_galerkin': (_c_i32, [_c_i64] + [_c_vp] * 11),
     'esr_mg_features': (_c_i32, [_c_i64] + [_c_vp] * 9),
+    'esr_mg_parents': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
     'esr_mg_restrict': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_mg_prolong': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_mg_jacobi_pre': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl,
@@ -95,6 +96,7 @@ class Level:
         self.rowptr, self.col, self.Ae, self.Av = rowptr, col, Ae, Av
         self.fpos, self.fidx = fpos, fidx
         self.Ainv = None
+        self.par = None          # parent map into the next coarser level
         self.graph = None        # network inputs (learned smoothing)
         self.bufs = None
 
@@ -161,6 +163,10 @@ class MultigridPreconditioner:
                      f.col.data_ptr(), f.Ae.data_ptr(), f.Av.data_ptr(), c_rowptr.data_ptr(),
                      c_Ae.data_ptr(), c_Av.data_ptr(), bad.data_ptr(), sp)
             self.levels.append(Level(nc, c_rowptr, c_col, c_Ae, c_Av))
+        for L in self.levels[:-1]:
+            L.par = torch.empty(2 * L.nv, dtype=torch.int32, device=dev)
+            nat.call('esr_mg_parents', L.n_grid, None if L.fpos is None else L.fpos.data_ptr(),
+                     L.par.data_ptr(), sp)
         if int(bad.item()):
             raise ValueError('the operator couples vertices that are not neighbours on the '
                              '2-triangle mesh: geometric coarsening does not apply')
@@ -212,7 +218,6 @@ class MultigridPreconditioner:
                          vfeat.data_ptr(), efeat.data_ptr(), nat.stream_ptr())
                 L.graph = _Graph(L.nv, L.rowptr, L.col, efeat, vfeat, L.Av)
             L.bufs = GnnBuffers(L.nv, dev)
-            L.y = torch.zeros(L.nv, dtype=torch.float64, device=dev)
             L.s = torch.zeros(L.nv, dtype=torch.float64, device=dev)
 
     # -- cycle ----------------------------------------------------------------
@@ -242,13 +247,12 @@ class MultigridPreconditioner:
             nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(),
                      C.r.data_ptr(), sp)
             self._cycle(k + 1, C.r, C.zo)
-            nat.call('esr_mg_jacobi_post', F64, L.n_grid, P(L.fpos), *csr, self.omega,
-                     r.data_ptr(), L.z.data_ptr(), C.zo.data_ptr(), out.data_ptr(), sp)
+            self._post(L, C, csr, r, out)
             return
         S = self.smoother
         if self.learned_pre:
             S.forward_into(L.graph, L.bufs, r, L.z)
-            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
+            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, None, *csr, r.data_ptr(),
                      L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
         else:
             nat.call('esr_mg_jacobi_pre', F64, L.nv, *csr, self.omega, r.data_ptr(),
@@ -257,10 +261,19 @@ class MultigridPreconditioner:
                  sp)
         self._cycle(k + 1, C.r, C.zo)
         if self.learned_post:
-            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, P(L.fpos), *csr, r.data_ptr(),
-                     L.z.data_ptr(), C.zo.data_ptr(), L.y.data_ptr(), L.t.data_ptr(), sp)
+            nat.call('esr_mg_prolong', F64, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
+                     L.z.data_ptr(), sp)
+            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, None, *csr, r.data_ptr(),
+                     L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
             S.forward_into(L.graph, L.bufs, L.t, L.s)
-            ops.ewise('add', L.y, L.s, out=out)
+            ops.ewise('add', L.z, L.s, out=out)
         else:
-            nat.call('esr_mg_jacobi_post', F64, L.n_grid, P(L.fpos), *csr, self.omega,
-                     r.data_ptr(), L.z.data_ptr(), C.zo.data_ptr(), out.data_ptr(), sp)
+            self._post(L, C, csr, r, out)
+
+    def _post(self, L, C, csr, r, out):
+        """z += P zc (in place), then out = z + w (r - A z) / d."""
+        sp = nat.stream_ptr()
+        nat.call('esr_mg_prolong', nat.F64, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
+                 L.z.data_ptr(), sp)
+        nat.call('esr_mg_jacobi_post', nat.F64, L.n_grid, None, *csr, self.omega,
+                 r.data_ptr(), L.z.data_ptr(), None, out.data_ptr(), sp)

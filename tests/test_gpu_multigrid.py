@@ -92,7 +92,20 @@ def test_transfers_bit_exact(gpu):
     nat.call('esr_mg_restrict', nat.F64, n, fidx_d.data_ptr(), t_d.data_ptr(),
              rc.data_ptr(), nat.stream_ptr())
     np.testing.assert_array_equal(rc.cpu().numpy(), omg.restrict(t, n, fidx))
-    nat.call('esr_mg_prolong', nat.F64, n, fpos_d.data_ptr(), zc_d.data_ptr(),
+    par = torch.empty(2 * fpos.size, dtype=torch.int32, device=gpu)
+    nat.call('esr_mg_parents', n, fpos_d.data_ptr(), par.data_ptr(), nat.stream_ptr())
+    # the parent map: interior coarse parents (-1 for boundary ones), -2 as
+    # the second entry of a vertex on a coarse vertex
+    pm = par.cpu().numpy().reshape(-1, 2)
+    Wf, nc = n + 1, n // 2
+    for i in range(fpos.size):
+        I, J = fpos[i] % Wf, fpos[i] // Wf
+        want = [b * (nc + 1) + a if 0 < a < nc and 0 < b < nc else -1
+                for a, b, _ in omg.parents(I, J)]
+        if len(want) == 1:
+            want.append(-2)
+        assert list(pm[i]) == want, (i, I, J, pm[i], want)
+    nat.call('esr_mg_prolong', nat.F64, n, par.data_ptr(), zc_d.data_ptr(),
              zd.data_ptr(), nat.stream_ptr())
     np.testing.assert_array_equal(zd.cpu().numpy(), z + omg.prolong_values(zc, n, fpos))
     # lexicographic fine level (no maps)
