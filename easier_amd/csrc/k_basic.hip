@@ -88,12 +88,66 @@ __global__ __launch_bounds__(kBlock) void k_gather_units(const U *__restrict__ s
   }
 }
 
+// One-unit rows, V consecutive outputs per vector (V units = 16 bytes): a
+// lane loads V indices with one load, gathers V rows and writes them with
+// one 16-byte store; two vectors per lane, kBlock vectors apart. The n % V
+// tail is written by the lane holding vector index n / V.
+typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
+typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+template <int V> struct GatherIdx;
+template <> struct GatherIdx<2> { typedef i32x2_t I; };
+template <> struct GatherIdx<4> { typedef i32x4_t I; };
+
+template <typename U, int V>
+__global__ __launch_bounds__(kBlock) void k_gather_vec(const U *__restrict__ src,
+                                                      const int32_t *__restrict__ idx,
+                                                      U *__restrict__ dst, int64_t n) {
+  static_assert(sizeof(U) * V == 16, "16-byte output vectors");
+  typedef typename GatherIdx<V>::I IV;
+  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t nvec = n / V;
+  int64_t vi[2];
+  IV ii[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    vi[k] = tile * (2 * kBlock) + k * kBlock + threadIdx.x;
+    if (vi[k] < nvec) ii[k] = __builtin_nontemporal_load(reinterpret_cast<const IV *>(idx) + vi[k]);
+  }
+  U v[2][V];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (vi[k] < nvec) {
+#pragma unroll
+      for (int c = 0; c < V; ++c) v[k][c] = src[ii[k][c]];
+    }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (vi[k] < nvec) {
+      u32x4_t w;
+      __builtin_memcpy(&w, v[k], 16);
+      __builtin_nontemporal_store(w, reinterpret_cast<u32x4_t *>(dst) + vi[k]);
+    } else if (vi[k] == nvec) {
+      for (int64_t e = nvec * V; e < n; ++e) dst[e] = src[idx[e]];
+    }
+  }
+}
+
 template <typename U>
 static int launch_gather(const void *src, const int32_t *idx, void *dst,
                          int64_t n_out, int64_t upr, hipStream_t s) {
   const int64_t total = n_out * upr;
   const int64_t grid = ceil_div(total, 4 * kBlock);
-  if (upr == 1)
+  constexpr int V = 16 / (int)sizeof(U);
+  const bool vec = upr == 1 && (V == 2 || V == 4) &&
+                   (uintptr_t)dst % 16 == 0 && (uintptr_t)idx % (4 * V) == 0;
+  if (vec) {
+    if constexpr (V == 2 || V == 4) {
+      const int64_t vgrid = ceil_div(n_out / V + 1, 2 * kBlock);
+      k_gather_vec<U, V><<<dim3((unsigned)vgrid), dim3(kBlock), 0, s>>>(
+          (const U *)src, idx, (U *)dst, n_out);
+    }
+  } else if (upr == 1)
     k_gather_row<U><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(
         (const U *)src, idx, (U *)dst, n_out);
   else
@@ -168,6 +222,66 @@ __global__ __launch_bounds__(kBlock) void k_csr_seq(const T *__restrict__ vals, 
   out[t] = acc;
 }
 
+// One-value rows without a permutation (the compiled Reducer's usual form):
+// each wavefront owns 64 consecutive rows, loads their whole entry segment
+// [rowptr[r0], rowptr[r0 + 64]) with contiguous (coalesced) loads into LDS,
+// then every lane folds its own row from LDS in entry order -- the same
+// sequential sum as k_csr_seq, with a third of the address work for rows of
+// ~3 entries. Segments longer than kCap entries fall back to direct loads.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ vals,
+                                                      const int32_t *__restrict__ rowptr,
+                                                      int64_t n_rows, int keep_empty,
+                                                      T *__restrict__ out) {
+  constexpr int kCap = 512, kU = 4;
+  __shared__ T stage[kBlock / 64][kCap];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t r0 = (tile * (kBlock / 64) + w) * 64;
+  if (r0 >= n_rows) return;                       // the whole wavefront
+  const int64_t r = r0 + lane;
+  const bool ok = r < n_rows;
+  const int64_t rend = r0 + 64 < n_rows ? r0 + 64 : n_rows;
+  const int32_t s0 = rowptr[r0], s1 = rowptr[rend];
+  const int32_t b = ok ? rowptr[r] : 0, e = ok ? rowptr[r + 1] : 0;
+  const int32_t len = s1 - s0;
+  T *st = stage[w];
+  const bool staged = len <= kCap;                // uniform over the wavefront
+  if (staged) {
+    // kU loads per lane in flight together (64 kU entries), then the rest
+    T reg[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int32_t k = lane + 64 * u;
+      reg[u] = k < len ? vals[(int64_t)s0 + k] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int32_t k = lane + 64 * u;
+      if (k < len) st[k] = reg[u];
+    }
+    for (int32_t k = lane + 64 * kU; k < len; k += 64) st[k] = vals[(int64_t)s0 + k];
+    // LDS written and read by lanes of one wavefront: order the accesses
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (!ok || e - b > ESR_SEQ_ROW_MAX) return;     // long rows: k_csr_long
+  if (b == e) {
+    if (!keep_empty) out[r] = T(0);
+    return;
+  }
+  RedOp<T, OP> op;
+  T acc = RedOp<T, OP>::identity();
+  if (staged) {
+    for (int32_t j = b; j < e; ++j) acc = op(acc, st[j - s0]);
+  } else {
+    for (int32_t j = b; j < e; ++j) acc = op(acc, vals[j]);
+  }
+  if constexpr (OP == ESR_RED_MEAN) acc = acc / T(e - b);
+  out[r] = acc;
+}
+
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_csr_long(const T *__restrict__ vals, int64_t width,
                                                     const int32_t *__restrict__ rowptr,
@@ -195,7 +309,11 @@ static int launch_csr(const void *vals, int64_t width, const int32_t *rowptr,
                       const int32_t *perm, int64_t n_rows, const int32_t *long_rows,
                       int64_t n_long, int keep_empty, void *out, hipStream_t s) {
   const int64_t total = n_rows * width;
-  if (total > 0) {
+  if (total > 0 && width == 1 && !perm) {
+    k_csr_staged<T, OP><<<dim3((unsigned)ceil_div(n_rows, kBlock)), dim3(kBlock), 0, s>>>(
+        (const T *)vals, rowptr, n_rows, keep_empty, (T *)out);
+    ESR_LAUNCHED();
+  } else if (total > 0) {
     k_csr_seq<T, OP><<<dim3((unsigned)ceil_div(total, kBlock)), dim3(kBlock), 0, s>>>(
         (const T *)vals, width, rowptr, perm, n_rows, keep_empty, (T *)out);
     ESR_LAUNCHED();

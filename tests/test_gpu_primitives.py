@@ -43,6 +43,13 @@ def test_gather_empty_and_large(gpu):
     idx = torch.randint(0, 10, (n,), generator=torch.Generator().manual_seed(1))
     out = ops.gather(src.to(gpu), idx.to(torch.int32).to(gpu))
     assert torch.equal(out.cpu(), src[idx])
+    # every tail length of the 16-byte vector path (2 float64 / 4 float32 per vector)
+    for dt in (torch.float64, torch.float32):
+        s = src.to(dt)
+        for n in range(1, 12):
+            idx = torch.randint(0, 10, (n,), generator=torch.Generator().manual_seed(n))
+            out = ops.gather(s.to(gpu), idx.to(torch.int32).to(gpu))
+            assert torch.equal(out.cpu(), s[idx]), (dt, n)
 
 
 @pytest.mark.parametrize('reduce', ['sum', 'prod', 'mean', 'amax', 'amin'])
@@ -59,6 +66,34 @@ def test_reducer_bit_exact(gpu, reduce, dtype):
     out = ops.csr_reduce(t.to(gpu), rowptr, perm, lr, n, reduce).cpu()
     ref = oops.reducer(t, idx, n, reduce)
     assert torch.equal(torch.nan_to_num(out, nan=7.0), torch.nan_to_num(ref, nan=7.0))
+
+
+@pytest.mark.parametrize('reduce', ['sum', 'prod', 'mean', 'amax', 'amin'])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_reducer_sorted_single_value_rows(gpu, reduce, dtype):
+    """Sorted index, one value per edge (the compiled Reducer's form, staged
+    through LDS per 64 rows): short rows, empty rows, a 64-row group whose
+    entries exceed the staging capacity, rows beyond the sequential limit."""
+    print(f'reduce sorted {reduce} {dtype}', flush=True)
+    g = torch.Generator().manual_seed(6)
+    n = 3001
+    counts = torch.randint(0, 7, (n,), generator=g)
+    counts[:64] = 0
+    counts[200:260] = 40                                 # 2,400 entries in one group
+    counts[1000] = 300                                   # beyond the sequential limit
+    idx = torch.repeat_interleave(torch.arange(n), counts)
+    t = torch.randn(idx.shape[0], generator=g, dtype=torch.float64).to(dtype)
+    if reduce in ('amax', 'amin'):
+        t[5] = float('nan')
+    rowptr = torch.from_numpy(np.concatenate([[0], np.cumsum(counts.numpy())]).astype(np.int32))
+    lr = torch.nonzero(counts > nat.SEQ_ROW_MAX).reshape(-1).to(torch.int32)
+    out = ops.csr_reduce(t.to(gpu), rowptr.to(gpu), None, lr.to(gpu), n, reduce).cpu()
+    ref = oops.reducer(t, idx, n, reduce)
+    keep = counts <= nat.SEQ_ROW_MAX
+    assert torch.equal(torch.nan_to_num(out[keep], nan=7.0), torch.nan_to_num(ref[keep], nan=7.0))
+    if reduce == 'sum':
+        assert abs(float(out[1000]) - math.fsum(t[idx == 1000].double().tolist())) <= \
+            1e-5 * float(t[idx == 1000].double().abs().sum())
 
 
 def test_reducer_out_keeps_empty_rows(gpu):
