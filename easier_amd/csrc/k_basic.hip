@@ -228,12 +228,14 @@ __global__ __launch_bounds__(kBlock) void k_csr_seq(const T *__restrict__ vals, 
 // then every lane folds its own row from LDS in entry order -- the same
 // sequential sum as k_csr_seq, with a third of the address work for rows of
 // ~3 entries. Segments longer than kCap entries fall back to direct loads.
-template <typename T, int OP>
+template <typename T, int OP, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ vals,
                                                       const int32_t *__restrict__ rowptr,
                                                       int64_t n_rows, int keep_empty,
                                                       T *__restrict__ out) {
   constexpr int kCap = 512, kU = 4;
+  constexpr int V = VEC ? 16 / (int)sizeof(T) : 1;   // values per staged load
+  typedef T tv_t __attribute__((ext_vector_type(V)));
   __shared__ T stage[kBlock / 64][kCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -243,24 +245,41 @@ __global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ val
   const bool ok = r < n_rows;
   const int64_t rend = r0 + 64 < n_rows ? r0 + 64 : n_rows;
   const int32_t s0 = rowptr[r0], s1 = rowptr[rend];
-  const int32_t b = ok ? rowptr[r] : 0, e = ok ? rowptr[r + 1] : 0;
-  const int32_t len = s1 - s0;
+  const int32_t b = ok ? rowptr[r] : 0;
+  const int32_t bn = __shfl_down(b, 1, 64);       // rowptr[r + 1] of the next lane
+  const int32_t e = ok ? (r + 1 < rend ? bn : s1) : 0;
+  // staged entries [a0, a0 + len): a0 = s0 rounded down to a 16-byte boundary
+  const int32_t a0 = s0 & ~(V - 1);
+  const int32_t nv = (s1 - a0 + V - 1) / V, len = nv * V;
   T *st = stage[w];
   const bool staged = len <= kCap;                // uniform over the wavefront
   if (staged) {
-    // kU loads per lane in flight together (64 kU entries), then the rest
-    T reg[kU];
+    const int32_t nnz = rowptr[n_rows];
+    auto load = [&](int32_t k) -> tv_t {           // vector k: entries a0 + kV ..
+      const int64_t j = (int64_t)a0 + (int64_t)k * V;
+      if (j + V <= nnz) return *reinterpret_cast<const tv_t *>(vals + j);
+      tv_t x;
+#pragma unroll
+      for (int c = 0; c < V; ++c) x[c] = j + c < nnz ? vals[j + c] : T(0);
+      return x;
+    };
+    auto put = [&](int32_t k, const tv_t &x) {
+#pragma unroll
+      for (int c = 0; c < V; ++c) st[k * V + c] = x[c];
+    };
+    // kU loads per lane in flight together, then the rest
+    tv_t reg[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int32_t k = lane + 64 * u;
-      reg[u] = k < len ? vals[(int64_t)s0 + k] : T(0);
+      if (k < nv) reg[u] = load(k);
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int32_t k = lane + 64 * u;
-      if (k < len) st[k] = reg[u];
+      if (k < nv) put(k, reg[u]);
     }
-    for (int32_t k = lane + 64 * kU; k < len; k += 64) st[k] = vals[(int64_t)s0 + k];
+    for (int32_t k = lane + 64 * kU; k < nv; k += 64) put(k, load(k));
     // LDS written and read by lanes of one wavefront: order the accesses
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -274,12 +293,12 @@ __global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ val
   RedOp<T, OP> op;
   T acc = RedOp<T, OP>::identity();
   if (staged) {
-    for (int32_t j = b; j < e; ++j) acc = op(acc, st[j - s0]);
+    for (int32_t j = b; j < e; ++j) acc = op(acc, st[j - a0]);
   } else {
     for (int32_t j = b; j < e; ++j) acc = op(acc, vals[j]);
   }
   if constexpr (OP == ESR_RED_MEAN) acc = acc / T(e - b);
-  out[r] = acc;
+  __builtin_nontemporal_store(acc, out + r);
 }
 
 template <typename T, int OP>
@@ -310,8 +329,13 @@ static int launch_csr(const void *vals, int64_t width, const int32_t *rowptr,
                       int64_t n_long, int keep_empty, void *out, hipStream_t s) {
   const int64_t total = n_rows * width;
   if (total > 0 && width == 1 && !perm) {
-    k_csr_staged<T, OP><<<dim3((unsigned)ceil_div(n_rows, kBlock)), dim3(kBlock), 0, s>>>(
-        (const T *)vals, rowptr, n_rows, keep_empty, (T *)out);
+    const dim3 grid((unsigned)ceil_div(n_rows, kBlock));
+    if ((uintptr_t)vals % 16 == 0 && (sizeof(T) == 4 || sizeof(T) == 8))
+      k_csr_staged<T, OP, true><<<grid, dim3(kBlock), 0, s>>>((const T *)vals, rowptr, n_rows,
+                                                             keep_empty, (T *)out);
+    else
+      k_csr_staged<T, OP, false><<<grid, dim3(kBlock), 0, s>>>((const T *)vals, rowptr, n_rows,
+                                                              keep_empty, (T *)out);
     ESR_LAUNCHED();
   } else if (total > 0) {
     k_csr_seq<T, OP><<<dim3((unsigned)ceil_div(total, kBlock)), dim3(kBlock), 0, s>>>(
