@@ -240,8 +240,8 @@ class TwoGrid:
         nat.call('esr_mg_restrict', nat.F64, L0.n_grid, None, t.data_ptr(),
                  self.rc.data_ptr(), sp)
         self.coarse_solve()
-        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, None, *self._csr(L0),
-                 t.data_ptr(), self.zero.data_ptr(), self.ec.data_ptr(),
+        nat.call('esr_mg_residual', nat.F64, L0.nv, L0.n_grid, L0.par.data_ptr(),
+                 *self._csr(L0), t.data_ptr(), self.zero.data_ptr(), self.ec.data_ptr(),
                  self.y.data_ptr(), self.aqw.data_ptr(), sp)
         return self.aqw
 
