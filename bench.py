@@ -180,15 +180,81 @@ def primitives(eqn, reps=20):
     t_g = run(lambda: ops.gather(src, sel.idx, out=edge_vals))
     t_r = run(lambda: ops.csr_reduce(edge_vals, red.easier_rowptr, red.easier_perm,
                                      red.easier_long_rows, red.n, 'sum', out=out_cells))
+    # aggregators (esr.sum / esr.norm of a cell tensor): two fixed-shape stages
+    col = src.reshape(-1, 1)
+    agg_out = torch.empty(1, dtype=src.dtype, device=src.device)
+    ws = torch.empty(ops.aggregate_ws_bytes(col), dtype=torch.uint8, device=src.device)
+    t_s = run(lambda: ops.aggregate('sum', col, out=agg_out, ws=ws))
+    t_n = run(lambda: ops.aggregate('norm', col, out=agg_out, ws=ws))
     ne, nc = sel.idx.shape[0], src.shape[0]
     bg = ne * (I + T) + nc * T
     br = ne * T + (nc + 1) * I + nc * T
+    ba = nc * T
     return {
         'selector_gather': dict(edges=ne, seconds=t_g, bytes=bg, GBps=bg / t_g / 1e9,
                                 frac=bg / t_g / HBM_PEAK),
         'reducer_csr_sum': dict(rows=nc, entries=ne, seconds=t_r, bytes=br,
                                 GBps=br / t_r / 1e9, frac=br / t_r / HBM_PEAK),
+        'aggregate_sum': dict(n=nc, seconds=t_s, bytes=ba, GBps=ba / t_s / 1e9,
+                              frac=ba / t_s / HBM_PEAK),
+        'aggregate_norm': dict(n=nc, seconds=t_n, bytes=ba, GBps=ba / t_n / 1e9,
+                               frac=ba / t_n / HBM_PEAK),
     }
+
+
+def poisson_gmres_config1(skip_cpu):
+    """GMRES(20), Jacobi-preconditioned, on the config-1 Poisson system:
+    the fused Arnoldi cycles against the oracle's GMRES on the host."""
+    import torch
+    import easier_amd as esr
+    from easier_amd.numeric.solver import GMRES
+    from easier_amd.workloads import poisson as wp
+    mesh, sys_ = wp.config1_system(100, seed=0)
+    eqn = wp.PoissonP1(sys_)
+    sol = GMRES(eqn.A, eqn.b, eqn.x, eqn.M, restart=20)
+    esr.compile([eqn, sol], 'hip')
+    times, res = [], None
+    for rep in range(4):
+        eqn.x.data.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sol.solve(rtol=1e-8, maxiter=500)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    out = dict(workload='P1 Poisson 100x100 (10,201 vertices), Jacobi-preconditioned '
+               'GMRES(20), FP64, rtol 1e-8', solve_ms=1e3 * sorted(times[1:])[1],
+               cycles=res['iters'], residual=res['residual'])
+    if not skip_cpu:
+        from oracle import solvers as osol
+        A = osol.Linsys(torch.from_numpy(sys_['Av']), torch.from_numpy(sys_['Ae']),
+                        torch.from_numpy(sys_['col']), torch.from_numpy(sys_['row']),
+                        sys_['Av'].shape[0])
+        b = torch.from_numpy(sys_['b'])
+        x = torch.zeros_like(b)
+        t0 = time.perf_counter()
+        info, _ = osol.gmres(A, b, x, M=lambda r: r / A.Av, restart=20, rtol=1e-8,
+                             maxiter=500)
+        out['cpu_solve_ms'] = 1e3 * (time.perf_counter() - t0)
+        out['cpu_cycles'] = info['iters']
+    return out
+
+
+def swe_fp32(n, seed, steps=100, warmup=5):
+    """The same SWE step in float32 (cells/s; parity 1e-5 in the tests)."""
+    import torch
+    mesh, geo, eqn, meta = build_swe(n, torch.float32, seed)
+    for _ in range(warmup):
+        eqn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eqn()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nc = mesh['cells'].shape[0]
+    return dict(workload=f'shallow-water RK4 {n}x{n}, float32', steps=steps,
+                ms_per_step=1e3 * wall / steps, cells_per_s=nc * steps / wall,
+                state_finite=bool(torch.isfinite(eqn.h.collect()).all()))
 
 
 def committed_traffic(n, dtype, kernel):
@@ -352,6 +418,8 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=2)
     ap.add_argument('--skip-cpu', action='store_true')
     ap.add_argument('--skip-poisson', action='store_true')
+    ap.add_argument('--skip-fp32', action='store_true',
+                    help='skip the float32 SWE side figure')
     ap.add_argument('--no-fuse', action='store_true')
     ap.add_argument('--profile', action='store_true',
                     help='only warm-up + timed steps (for rocprofv3 runs)')
@@ -447,6 +515,13 @@ def main():
     if not args.skip_poisson:
         log('Poisson config 1')
         result['poisson_config1'] = poisson_config1(args.skip_cpu)
+        log('Poisson config 1, GMRES(20)')
+        result['poisson_gmres_config1'] = poisson_gmres_config1(args.skip_cpu)
+    if not args.skip_fp32 and args.dtype == 'f64':
+        log('SWE float32')
+        del eqn
+        torch.cuda.empty_cache()
+        result['swe_fp32'] = swe_fp32(args.n, args.seed)
     extra = {c.strip() for c in args.configs.split(',') if c.strip()}
     if '3' in extra:
         result['batched_config3'] = batched_config3()

@@ -443,8 +443,18 @@ __global__ __launch_bounds__(kBlock) void k_agg_stage1(const T *__restrict__ x, 
   const int64_t lo = b * chunk, hi = lo + chunk < n ? lo + chunk : n;
   AggCombine<T> cmb{op, p};
   T acc = cmb.identity();
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock)
-    acc = cmb(acc, agg_pre<T>(op, p, x[i * width + c]));
+  // four loads in flight per step, folded in the same per-thread order as
+  // the plain strided loop (i, i + kBlock, ...)
+  int64_t i = lo + threadIdx.x;
+  for (; i + 3 * kBlock < hi; i += 4 * kBlock) {
+    const T v0 = x[i * width + c], v1 = x[(i + kBlock) * width + c],
+            v2 = x[(i + 2 * kBlock) * width + c], v3 = x[(i + 3 * kBlock) * width + c];
+    acc = cmb(acc, agg_pre<T>(op, p, v0));
+    acc = cmb(acc, agg_pre<T>(op, p, v1));
+    acc = cmb(acc, agg_pre<T>(op, p, v2));
+    acc = cmb(acc, agg_pre<T>(op, p, v3));
+  }
+  for (; i < hi; i += kBlock) acc = cmb(acc, agg_pre<T>(op, p, x[i * width + c]));
   acc = block_reduce(acc, cmb, cmb.identity());
   if (threadIdx.x == 0) part[c * nb + b] = acc;
 }
