@@ -76,8 +76,15 @@ class Unsupported(Exception):
 #                  no remap 1.285 / 1.189, xcd_remap 1.002 / 1.217,
 #                  G = 16 1.105 / 1.164, G = 64 1.035 / 1.175, G = 256
 #                  1.009 / 1.196.
+#   rows_per_thread - 2: a thread evaluates rows r0 and r0 + 1 (r0 even); the
+#                  row arrays are read and written as 2-element vectors (16
+#                  bytes in float64), the edge loops stay per row. Measured on
+#                  the 2048 x 2048 SWE step: 1.38 vs 1.18 ms (half the threads
+#                  leave fewer gathers in flight than the wider row accesses
+#                  save), so the default is 1.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
-           'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64}
+           'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64,
+           'rows_per_thread': 1}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -425,6 +432,38 @@ __device__ __forceinline__ i64 esr_remap(i64 b, i64 nb) {
   const i64 x = b % ESR_XCD, pos = b / ESR_XCD;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
 }
+// two consecutive rows as one vector access (rows r, r + 1; at the last row
+// of an odd count, a scalar access, the second lane a copy)
+typedef double esr_v2d __attribute__((ext_vector_type(2)));
+typedef float esr_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ esr_v2d esr_ld2(const double *p, i64 r, i64 n) {
+  if (r + 1 < n) return *reinterpret_cast<const esr_v2d *>(p + r);
+  esr_v2d v; v.x = p[r]; v.y = v.x; return v;
+}
+__device__ __forceinline__ esr_v2f esr_ld2(const float *p, i64 r, i64 n) {
+  if (r + 1 < n) return *reinterpret_cast<const esr_v2f *>(p + r);
+  esr_v2f v; v.x = p[r]; v.y = v.x; return v;
+}
+__device__ __forceinline__ void esr_st2(double *p, i64 r, i64 n, double x, double y) {
+  if (r + 1 < n) { esr_v2d v; v.x = x; v.y = y; *reinterpret_cast<esr_v2d *>(p + r) = v; }
+  else p[r] = x;
+}
+__device__ __forceinline__ void esr_st2(float *p, i64 r, i64 n, float x, float y) {
+  if (r + 1 < n) { esr_v2f v; v.x = x; v.y = y; *reinterpret_cast<esr_v2f *>(p + r) = v; }
+  else p[r] = x;
+}
+__device__ __forceinline__ void esr_st2nt(double *p, i64 r, i64 n, double x, double y) {
+  if (r + 1 < n) {
+    esr_v2d v; v.x = x; v.y = y;
+    __builtin_nontemporal_store(v, reinterpret_cast<esr_v2d *>(p + r));
+  } else __builtin_nontemporal_store(x, p + r);
+}
+__device__ __forceinline__ void esr_st2nt(float *p, i64 r, i64 n, float x, float y) {
+  if (r + 1 < n) {
+    esr_v2f v; v.x = x; v.y = y;
+    __builtin_nontemporal_store(v, reinterpret_cast<esr_v2f *>(p + r));
+  } else __builtin_nontemporal_store(x, p + r);
+}
 // correctly rounded sqrt: device sqrt, then the Tuckerman rounding test
 __device__ __forceinline__ double esr_sqrt(double x) {
   const double y = sqrt(x);
@@ -453,6 +492,10 @@ class _Restage(Exception):
     """The staged edge arrays of a kernel exceed the LDS budget."""
 
 
+class _RowsPerThread(Exception):
+    """An array of a two-rows-per-thread kernel is not vector-aligned."""
+
+
 class Kernel:
     def __init__(self, space, stage, name):
         self.space, self.stage, self.name = space, stage, name
@@ -464,6 +507,7 @@ class Kernel:
         self.staged = False
         self.lds_bytes = 0
         self.block = int(OPTIONS['block'])
+        self.rpt = 1                      # rows per thread
 
     def algorithmic_bytes(self) -> int:
         """Compulsory HBM bytes of one launch: every array the kernel reads
@@ -511,6 +555,7 @@ class _Lowering:
         self.device = prog.device
         self.kernels: List[Kernel] = []
         self.no_stage = False
+        self.force_rpt1 = False
 
     def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
         a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
@@ -542,7 +587,14 @@ class _Lowering:
             kern = Kernel(space, stage, f'esr_fused_{k}')
             kern.n = self.prog.space_len[space]
             try:
-                self._emit_kernel(kern, roots)
+                try:
+                    self._emit_kernel(kern, roots)
+                except _RowsPerThread:
+                    self.force_rpt1 = True
+                    kern = Kernel(space, stage, f'esr_fused_{k}')
+                    kern.n = self.prog.space_len[space]
+                    self._emit_kernel(kern, roots)
+                    self.force_rpt1 = False
             except _Restage:
                 # too many staged edge arrays for the LDS budget: this
                 # kernel reads its edges straight from global memory
@@ -573,9 +625,24 @@ class _Lowering:
 
     def _emit_kernel(self, kern: Kernel, roots: List[V]):
         ct = self.ctype
-        rows: List[str] = []
-        memo: Dict[int, str] = {}
-        done_groups = set()
+        # two rows per thread (rows r0, r0 + 1): the row bodies are emitted
+        # once per row, their row-indexed loads and stores shared as one
+        # 2-element vector access; LDS edge staging keeps one row per thread
+        rpt = 2 if (OPTIONS['rows_per_thread'] == 2 and not self.force_rpt1 and
+                    not (OPTIONS['stage_edges'] and not self.no_stage)) else 1
+        kern.rpt = rpt
+        vt = 'esr_v2d' if ct == 'double' else 'esr_v2f'
+        prologue: List[str] = []
+        vloads: Dict[str, str] = {}
+
+        class _Cx:
+            rows: List[str]
+            memo: Dict[int, str]
+            groups: set
+            R = 'r'
+            comp = None
+            sfx = ''
+        cx = _Cx()
 
         def buf_of(v: V):
             if v.kind == 'leaf':
@@ -584,6 +651,15 @@ class _Lowering:
                 return self._param(kern, v.obj.data, ct)
             assert v.mat and v.buf is not None
             return self._param(kern, v.buf, ct)
+
+        def row_load(buf):
+            """buf[row] in the current row context."""
+            if cx.comp is None:
+                return var(cx.rows, f'{buf}[{cx.R}]')
+            if buf not in vloads:
+                vloads[buf] = f'w{len(vloads)}'
+                prologue.append(f'  const {vt} {vloads[buf]} = esr_ld2({buf}, r0, a.n);')
+            return f'{vloads[buf]}.{cx.comp}'
 
         def var(stmts, expr, typ=None):
             name = f'v{len(memo_names)}'
@@ -629,39 +705,39 @@ class _Lowering:
             return v.kind == 'ew' and v.op in _PRED
 
         def emit_row(v: V) -> str:
-            if v.id in memo:
-                return memo[v.id]
+            if v.id in cx.memo:
+                return cx.memo[v.id]
             if v.kind == 'const':
                 s = _fmt_const(v.const, ct)
             elif v.kind == 'scalar':
-                s = var(rows, f'{buf_of(v)}[0]')
+                s = var(cx.rows, f'{buf_of(v)}[0]')
             elif v.kind == 'leaf' or (v.mat and v.stage < kern.stage) or \
                     (v.mat and v.space != kern.space):
                 if v.space != kern.space:
                     raise Unsupported('row read from another space')
-                s = var(rows, f'{buf_of(v)}[r]')
+                s = row_load(buf_of(v))
             elif v.kind == 'gather':
                 sel = v.obj
                 src = v.args[0]
                 idx = self._param(kern, sel.idx, 'int')
-                s = var(rows, f'{buf_of(src)}[{idx}[r]]')
+                s = var(cx.rows, f'{buf_of(src)}[{idx}[{cx.R}]]')
             elif v.kind == 'reduce':
                 emit_group(v)
-                s = memo[v.id]
+                s = cx.memo[v.id]
             elif v.kind == 'ew':
                 a = [emit_row(x) for x in v.args]
-                s = var(rows, ew_expr(v, a), 'bool' if is_pred(v) else None)
+                s = var(cx.rows, ew_expr(v, a), 'bool' if is_pred(v) else None)
             else:
                 raise Unsupported(v.kind)
-            memo[v.id] = s
+            cx.memo[v.id] = s
             return s
 
         def emit_group(v: V):
             red = v.obj
             key = (id(red),)
-            if key in done_groups:
+            if key in cx.groups:
                 return
-            done_groups.add(key)
+            cx.groups.add(key)
             members = [u for u in self.p.values if u.kind == 'reduce' and u.obj is red
                        and u.stage == kern.stage and u.space == kern.space
                        and (u.id in needed)]
@@ -684,7 +760,7 @@ class _Lowering:
                 self._param(kern, red.easier_perm, 'int')
             body: List[str] = []
             emem: Dict[int, str] = {}
-            gid = len(done_groups)
+            gid = len(cx.groups)
             # edge arrays staged through LDS: param name -> (lds name, ctype)
             staged: Dict[str, tuple] = {}
             stage = dense and perm is None and OPTIONS['stage_edges'] and not self.no_stage
@@ -714,7 +790,7 @@ class _Lowering:
                 if d is not None:
                     dn = self._param(kern, d, 'short')
                     hi = self.prog.space_len[src.space] - 1
-                    s = var(body, f'esr_clampi((int)r + (int){edge_load(dn, "short")}, {hi})',
+                    s = var(body, f'esr_clampi((int){cx.R} + (int){edge_load(dn, "short")}, {hi})',
                             'int')
                 else:
                     idx = self._param(kern, sel.idx, 'int')
@@ -749,9 +825,9 @@ class _Lowering:
 
             accs = []
             for u in members:
-                accs.append(f'acc{u.id}')
+                accs.append(f'acc{u.id}{cx.sfx}')
             inner = [emit_edge(u.args[0]) for u in members]
-            rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
+            cx.rows.append('  ' + ' '.join(f'{ct} {a} = ({ct})0;' for a in accs))
             if stage:
                 cap = B * kmax
                 lds = sum(cap * (4 if c == 'int' else self.p.dtype.itemsize)
@@ -763,19 +839,19 @@ class _Lowering:
                 # the block's rows [rb, rb + B) own the contiguous edge
                 # range [eb, eb + nl), nl <= B * kmax; copy it coalesced
                 for name, (sn, c) in staged.items():
-                    rows.append(f'  __shared__ {c} {sn}[{cap}];')
-                rows.append(f'  const int eb{gid} = {rp}[rb], nl{gid} = '
+                    cx.rows.append(f'  __shared__ {c} {sn}[{cap}];')
+                cx.rows.append(f'  const int eb{gid} = {rp}[rb], nl{gid} = '
                             f'{rp}[rb + {B} < a.n ? rb + {B} : a.n] - eb{gid};')
                 ld = (lambda nm, i: f'__builtin_nontemporal_load(&{nm}[{i}])') if nt else \
                     (lambda nm, i: f'{nm}[{i}]')
-                rows.append(f'  for (int i = threadIdx.x; i < nl{gid}; i += {B}) {{')
+                cx.rows.append(f'  for (int i = threadIdx.x; i < nl{gid}; i += {B}) {{')
                 for name, (sn, c) in staged.items():
-                    rows.append(f'    {sn}[i] = {ld(name, f"eb{gid} + i")};')
-                rows.append('  }')
+                    cx.rows.append(f'    {sn}[i] = {ld(name, f"eb{gid} + i")};')
+                cx.rows.append('  }')
                 # an empty range still gives every slot a valid (row 0) entry
-                rows.append(f'  if (nl{gid} == 0 && threadIdx.x == 0) {{ ' +
+                cx.rows.append(f'  if (nl{gid} == 0 && threadIdx.x == 0) {{ ' +
                             ' '.join(f'{sn}[0] = 0;' for sn, _ in staged.values()) + ' }')
-                rows.append('  __syncthreads();')
+                cx.rows.append('  __syncthreads();')
             if dense:
                 # every row has at most kmax entries: evaluate kmax slots with
                 # clamped (always valid) edge positions so all their loads
@@ -783,39 +859,40 @@ class _Lowering:
                 # exists -- the same left-to-right order as the loop
                 for u, a in zip(members, accs):
                     body.append(f'  {a} = (j < je) ? {a} + {inner[members.index(u)]} : {a};')
-                rows.append(f'  {{ const int jb = {rp}[r], je = {rp}[r + 1];')
-                rows.append(f'  #pragma unroll')
-                rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
-                rows.append('    const int j = jb + k;')
+                cx.rows.append(f'  {{ const int jb = {rp}[{cx.R}], je = {rp}[{cx.R} + 1];')
+                cx.rows.append(f'  #pragma unroll')
+                cx.rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
+                cx.rows.append('    const int j = jb + k;')
                 if stage:
-                    rows.append(f'    int jl = j - eb{gid}; jl = jl < nl{gid} ? jl : nl{gid} - 1; '
+                    cx.rows.append(f'    int jl = j - eb{gid}; jl = jl < nl{gid} ? jl : nl{gid} - 1; '
                                 'jl = jl > 0 ? jl : 0;')
                 else:
-                    rows.append('    const int jc = j < je ? j : (je > 0 ? je - 1 : 0);')
-                    rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
-                rows.extend('  ' + b for b in body)
-                rows.append('  } }')
+                    cx.rows.append('    const int jc = j < je ? j : (je > 0 ? je - 1 : 0);')
+                    cx.rows.append(f'    const int e = {perm + "[jc]" if perm else "jc"};')
+                cx.rows.extend('  ' + b for b in body)
+                cx.rows.append('  } }')
             elif sparse:
                 # compact row c of the row: the set bits of the bitmap before it
                 for u, a in zip(members, accs):
                     body.append(f'  {a} = {a} + {inner[members.index(u)]};')
-                rows.append(f'  {{ const unsigned long long bw{gid} = {bmp}[r >> 6];')
-                rows.append(f'  if ((bw{gid} >> (r & 63)) & 1ull) {{')
-                rows.append(f'    const int c = {prp}[r >> 6] + '
-                            f'__popcll(bw{gid} & ((1ull << (r & 63)) - 1ull));')
-                rows.append(f'    for (int j = {rp}[c]; j < {rp}[c + 1]; ++j) {{')
-                rows.append(f'      const int e = {perm + "[j]" if perm else "j"};')
-                rows.extend('    ' + b for b in body)
-                rows.append('    } } }')
+                R = cx.R
+                cx.rows.append(f'  {{ const unsigned long long bw{gid} = {bmp}[{R} >> 6];')
+                cx.rows.append(f'  if ((bw{gid} >> ({R} & 63)) & 1ull) {{')
+                cx.rows.append(f'    const int c = {prp}[{R} >> 6] + '
+                               f'__popcll(bw{gid} & ((1ull << ({R} & 63)) - 1ull));')
+                cx.rows.append(f'    for (int j = {rp}[c]; j < {rp}[c + 1]; ++j) {{')
+                cx.rows.append(f'      const int e = {perm + "[j]" if perm else "j"};')
+                cx.rows.extend('    ' + b for b in body)
+                cx.rows.append('    } } }')
             else:
                 for u, a in zip(members, accs):
                     body.append(f'  {a} = {a} + {inner[members.index(u)]};')
-                rows.append(f'  for (int j = {rp}[r]; j < {rp}[r + 1]; ++j) {{')
-                rows.append(f'    const int e = {perm + "[j]" if perm else "j"};')
-                rows.extend('  ' + b for b in body)
-                rows.append('  }')
+                cx.rows.append(f'  for (int j = {rp}[{cx.R}]; j < {rp}[{cx.R} + 1]; ++j) {{')
+                cx.rows.append(f'    const int e = {perm + "[j]" if perm else "j"};')
+                cx.rows.extend('  ' + b for b in body)
+                cx.rows.append('  }')
             for u, a in zip(members, accs):
-                memo[u.id] = a
+                cx.memo[u.id] = a
 
         # values this kernel must compute (stop at values other kernels write)
         needed = set()
@@ -835,22 +912,44 @@ class _Lowering:
             for a in rt.args:
                 mark(a)
 
-        stores = []
-        for rt in roots:
-            if rt.kind == 'store':
-                val = rt.args[0]
-                if val.mat and val.stage != kern.stage:
-                    s = var(rows, f'{buf_of(val)}[r]')
+        contexts = [('r', None, '')] if rpt == 1 else [('ra', 'x', 'a'), ('rb1', 'y', 'b')]
+        ctx_stores = []
+        body_rows: List[str] = []
+        for R, comp, sfx in contexts:
+            cx.rows, cx.memo, cx.groups = [], {}, set()
+            cx.R, cx.comp, cx.sfx = R, comp, sfx
+            stores = []
+            for rt in roots:
+                if rt.kind == 'store':
+                    val = rt.args[0]
+                    if val.mat and val.stage != kern.stage:
+                        s = row_load(buf_of(val))
+                    else:
+                        s = emit_row(val)
+                    stores.append((self._param(kern, rt.obj.data, ct, const=False), s))
                 else:
-                    s = emit_row(val)
-                stores.append((self._param(kern, rt.obj.data, ct, const=False), s))
+                    s = emit_row(rt)
+                    stores.append((self._param(kern, rt.buf, ct, const=False), s))
+            ctx_stores.append(stores)
+            body_rows.extend(cx.rows)
+        rows = prologue + body_rows
+        nt_st = OPTIONS['nt_stores']
+        if rpt == 1:
+            if nt_st:
+                st = [f'  __builtin_nontemporal_store({s}, &{d}[r]);' for d, s in ctx_stores[0]]
             else:
-                s = emit_row(rt)
-                stores.append((self._param(kern, rt.buf, ct, const=False), s))
-        if OPTIONS['nt_stores']:
-            st = [f'  __builtin_nontemporal_store({s}, &{d}[r]);' for d, s in stores]
+                st = [f'  {d}[r] = {s};' for d, s in ctx_stores[0]]
         else:
-            st = [f'  {d}[r] = {s};' for d, s in stores]
+            fn = 'esr_st2nt' if nt_st else 'esr_st2'
+            st = [f'  {fn}({d}, r0, a.n, {sa}, {sb});'
+                  for (d, sa), (_, sb) in zip(ctx_stores[0], ctx_stores[1])]
+            # the paired accesses are 2-element vectors: their arrays must be
+            # aligned to the vector (else this kernel keeps one row per thread)
+            vec = 2 * self.p.dtype.itemsize
+            names = set(vloads) | {d for d, _ in ctx_stores[0]}
+            for cname, _, t in kern.params:
+                if cname in names and t.data_ptr() % vec:
+                    raise _RowsPerThread()
         params = ''.join(f'  {ctyp} __restrict__ p{i};\n'
                          for i, (_, ctyp, _) in enumerate(kern.params))
         if OPTIONS['xcd_group']:
@@ -859,7 +958,12 @@ class _Lowering:
             tile = 'esr_remap(blockIdx.x, gridDim.x)'
         else:
             tile = '(i64)blockIdx.x'
-        if kern.staged:
+        if rpt == 2:
+            head = (f'  const i64 r0 = ({tile} * {kern.block} + threadIdx.x) * 2;\n'
+                    '  if (r0 >= a.n) return;\n'
+                    '  const i64 ra = r0, rb1 = r0 + 1 < a.n ? r0 + 1 : r0;\n')
+            tail = '\n'.join(st)
+        elif kern.staged:
             # every thread of the block takes part in the LDS staging: rows
             # past the end recompute the last row and do not store
             head = (f'  const i64 rb = {tile} * {kern.block};\n'
@@ -925,7 +1029,7 @@ def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
         nat.call('esr_rtc_function', mod, k.name.encode(), ctypes.byref(fn))
         raw = low.pack(k)
         buf = (ctypes.c_char * len(raw)).from_buffer_copy(raw)
-        grid = max(1, (k.n + k.block - 1) // k.block)
+        grid = max(1, (k.n + k.block * k.rpt - 1) // (k.block * k.rpt))
         launches.append((fn, grid, buf, k.block))
     keep = [v.buf for v in p.values if v.buf is not None]
     eng = FusedEngine(m, launches, keep)

@@ -78,13 +78,17 @@ def test_sparse_row_bitmap_finds_every_row():
 
 @pytest.mark.parametrize('opts', [
     dict(stage_edges=False, nt_loads=False, nt_stores=False, block=256, xcd_remap=True,
-         xcd_group=0),
+         xcd_group=0, rows_per_thread=1),
     dict(stage_edges=True, nt_loads=False, nt_stores=True, block=128, xcd_remap=False,
-         xcd_group=0),
+         xcd_group=0, rows_per_thread=1),
     dict(stage_edges=True, nt_loads=True, nt_stores=False, block=512, xcd_remap=True,
-         xcd_group=0),
+         xcd_group=0, rows_per_thread=2),
     dict(stage_edges=False, nt_loads=False, nt_stores=True, block=256, xcd_remap=False,
-         xcd_group=64)])
+         xcd_group=64, rows_per_thread=1),
+    dict(stage_edges=False, nt_loads=False, nt_stores=True, block=256, xcd_remap=False,
+         xcd_group=64, rows_per_thread=2),
+    dict(stage_edges=False, nt_loads=False, nt_stores=False, block=128, xcd_remap=False,
+         xcd_group=0, rows_per_thread=2)])
 def test_code_generation_options(opts):
     mesh = wmesh.tri2(7)
     geo = oswe.geometry(mesh)
@@ -96,10 +100,15 @@ def test_code_generation_options(opts):
         p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
     finally:
         fuser.OPTIONS.update(saved)
+    ksrc = ''.join(k.src for k in low.kernels)
     assert all(k.block == opts['block'] for k in low.kernels)
     assert all(k.staged == opts['stage_edges'] for k in low.kernels)
-    assert ('__syncthreads' in src) == opts['stage_edges']
-    assert ('__builtin_nontemporal_store' in src) == opts['nt_stores']
+    # two rows per thread unless the edges are staged through LDS
+    rpt = 2 if opts['rows_per_thread'] == 2 and not opts['stage_edges'] else 1
+    assert all(k.rpt == rpt for k in low.kernels)
+    assert ('esr_ld2(' in ksrc) == (rpt == 2)
+    assert ('__syncthreads' in ksrc) == opts['stage_edges']
+    assert ('__builtin_nontemporal_store' in ksrc or 'esr_st2nt(' in ksrc) == opts['nt_stores']
     assert ('esr_remap(blockIdx.x' in src) == (opts['xcd_remap'] and not opts['xcd_group'])
     assert ('esr_group_remap(blockIdx.x' in src) == bool(opts['xcd_group'])
     # the staged edge range of one block fits its LDS arrays: 3 edges per row

@@ -157,6 +157,58 @@ def test_swe_steps_match_oracle(gpu, dtype, gen, g, fuse):
             assert torch.equal(got, want), k
 
 
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_two_rows_per_thread_kernels_are_bit_identical(gpu, dtype):
+    """The fused kernels evaluating two rows per thread (vector row
+    accesses) give the same bits as one row per thread: the SWE step (even
+    cell count) and a program on an odd number of rows (the last thread's
+    second row is past the end)."""
+    from easier_amd.core import fuser
+    print(f'two rows per thread {dtype}', flush=True)
+    mesh = wmesh.tri2(10)
+    geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
+    dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], 9.81)
+
+    class Odd(esr.Module):
+        def __init__(self, n=1001):
+            super().__init__()
+            g = torch.Generator().manual_seed(9)
+            src = torch.randint(0, n, (3 * n,), generator=g)
+            dst = torch.arange(3 * n) % n
+            self.red = esr.Reducer(dst, n)
+            self.sel = esr.Selector(src)
+            self.u = esr.Tensor(torch.rand(n, generator=g, dtype=torch.float64).to(dtype),
+                                mode='partition')
+            self.w = esr.Tensor(torch.rand(3 * n, generator=g, dtype=torch.float64).to(dtype),
+                                mode='partition')
+
+        def forward(self):
+            self.u[:] = self.u * 0.5 + self.red(self.sel(self.u) * self.w)
+
+    outs = []
+    saved = dict(fuser.OPTIONS)
+    try:
+        for rpt in (1, 2):
+            fuser.OPTIONS['rows_per_thread'] = rpt
+            eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=9.81, dtype=dtype)
+            odd = Odd()
+            esr.compile([eqn, odd], 'hip')
+            assert all(k.rpt == rpt for k in eqn.easier_engine.kernels)
+            for _ in range(5):
+                eqn()
+                odd()
+            outs.append([eqn.h.collect(), eqn.uh.collect(), eqn.vh.collect(), odd.u.collect()])
+    finally:
+        fuser.OPTIONS.update(saved)
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    if dtype == torch.float64:
+        ref = oswe.SWE(mesh, geo, dt=dt, g=9.81, dtype=dtype)
+        for _ in range(5):
+            ref()
+        assert torch.equal(outs[1][0], ref.h)
+
+
 def test_fused_cg_graph_replays_are_deterministic(gpu):
     eqn = Poisson5(40, seed=3)
     sol = CG(eqn.A, eqn.b, eqn.x, Jacobi(eqn.Ac))
