@@ -366,6 +366,24 @@ def mixed_config4(reps=1):
                bnorm=imx['bnorm'],
                rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64)))
     log(f'config 4: fp64 {t64:.3f} s ({i64["iters"]} it), mixed {tmx:.3f} s')
+    # the same problem with the multigrid preconditioner in flexible CG:
+    # float64 cycle, and float32 cycle (float64 dots, iteration, residual)
+    import torch
+    from easier_amd.numeric.preconditioner.multigrid import MultigridPreconditioner
+    from easier_amd.numeric.solver import FlexibleCG
+    logk = wp.gaussian_random_field(2048, 0).reshape(-1)
+    for tag, cd in (('mg_fp64', torch.float64), ('mg_fp32_cycle', torch.float32)):
+        e3 = wp.PoissonP1(sys_)
+        fc = FlexibleCG(e3.A, e3.b, e3.x, MultigridPreconditioner(None, cycle_dtype=cd),
+                        dict(points=mesh['points'], logk=logk, n_grid=2048))
+        esr.compile([e3, fc], 'hip')
+        tm, im = _best_time(lambda: fc.solve(rtol=1e-10, maxiter=1000), max(reps, 2),
+                            reset=lambda: e3.x.data.zero_())
+        xm = e3.x.collect().numpy()
+        out[tag] = dict(seconds=tm, iters=im['iters'], true_residual=im['residual'],
+                        rel_err_vs_fp64=float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)))
+        log(f'config 4 {tag}: {tm * 1e3:.2f} ms ({im["iters"]} it)')
+        del e3, fc
     return out
 
 

@@ -102,7 +102,11 @@ class Level:
 
 
 class MultigridPreconditioner:
-    """V(1,1)-cycle preconditioner. smoother: None (damped Jacobi, weight
+    """V(1,1)-cycle preconditioner. cycle_dtype float32 runs the whole cycle
+    (vectors, level operators, sweeps and transfers) in float32 on float32
+    copies of the Galerkin operators, built in float64: the mixed-precision
+    form in which flexible CG keeps dot products, the iteration and the
+    convergence test in float64. smoother: None (damped Jacobi, weight
     omega) or a LearnedPreconditioner (its network smooths every level with at
     least `learned_min_grid` cells per side, before the coarse-grid correction
     if learned_pre and after it if learned_post, damped Jacobi otherwise;
@@ -114,8 +118,14 @@ class MultigridPreconditioner:
     fcg_chunk = 4        # flexible-CG steps per captured graph
 
     def __init__(self, smoother=None, omega=0.8, coarsest=16, learned_min_grid=64,
-                 learned_pre=True, learned_post=False):
+                 learned_pre=True, learned_post=False, cycle_dtype=torch.float64):
+        if cycle_dtype not in (torch.float64, torch.float32):
+            raise ValueError('cycle_dtype must be float64 or float32')
+        if smoother is not None and cycle_dtype != torch.float64:
+            raise ValueError('the learned smoother runs in float64 cycles only')
         self.smoother = smoother
+        self.cycle_dtype = cycle_dtype
+        self.dt = nat.dtype_code(cycle_dtype)
         self.learned_pre, self.learned_post = bool(learned_pre), bool(learned_post)
         self.omega = float(omega)
         self.coarsest = int(coarsest)
@@ -177,12 +187,19 @@ class MultigridPreconditioner:
         np.add.at(dense, (rows, cl), last.Ae.cpu().numpy())
         dense[np.arange(last.nv), np.arange(last.nv)] += last.Av.cpu().numpy()
         last.Ainv = torch.from_numpy(np.linalg.inv(dense)).to(dev)
+        cd = self.cycle_dtype
         for k, L in enumerate(self.levels):
-            L.z = torch.zeros(L.nv, dtype=torch.float64, device=dev)
-            L.t = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+            # the operator the cycle reads (a float32 copy in a float32 cycle)
+            L.cAe = L.Ae if cd == torch.float64 else L.Ae.to(cd)
+            L.cAv = L.Av if cd == torch.float64 else L.Av.to(cd)
+            L.z = torch.zeros(L.nv, dtype=cd, device=dev)
+            L.t = torch.zeros(L.nv, dtype=cd, device=dev)
             if k > 0:
-                L.r = torch.zeros(L.nv, dtype=torch.float64, device=dev)
-                L.zo = torch.zeros(L.nv, dtype=torch.float64, device=dev)
+                L.r = torch.zeros(L.nv, dtype=cd, device=dev)
+                L.zo = torch.zeros(L.nv, dtype=cd, device=dev)
+        if cd != torch.float64:
+            self.r_in = torch.zeros(nv, dtype=cd, device=dev)
+            self.z_out = torch.zeros(nv, dtype=cd, device=dev)
         if self.smoother is not None:
             self._setup_learned(logk, graph0, fpos)
         MultigridPreconditioner._bindings += 1
@@ -224,7 +241,12 @@ class MultigridPreconditioner:
     def apply(self, r: torch.Tensor, z: torch.Tensor):
         if not self.levels:
             raise RuntimeError('MultigridPreconditioner.setup() first')
-        self._cycle(0, r, z)
+        if self.cycle_dtype == torch.float64:
+            self._cycle(0, r, z)
+        else:
+            ops.ewise('copy', r, out=self.r_in)
+            self._cycle(0, self.r_in, self.z_out)
+            ops.ewise('copy', self.z_out, out=z)
         return z
 
     def __call__(self, r):
@@ -233,18 +255,18 @@ class MultigridPreconditioner:
     def _cycle(self, k, r, out):
         L = self.levels[k]
         sp = nat.stream_ptr()
-        F64 = nat.F64
+        DT = self.dt           # the cycle's dtype code
         if L.Ainv is not None:
-            nat.call('esr_mg_dense', F64, L.nv, L.Ainv.data_ptr(), r.data_ptr(),
+            nat.call('esr_mg_dense', DT, L.nv, L.Ainv.data_ptr(), r.data_ptr(),
                      out.data_ptr(), sp)
             return
         C = self.levels[k + 1]
         P = lambda t: None if t is None else t.data_ptr()
-        csr = (L.rowptr.data_ptr(), L.col.data_ptr(), L.Ae.data_ptr(), L.Av.data_ptr())
+        csr = (L.rowptr.data_ptr(), L.col.data_ptr(), L.cAe.data_ptr(), L.cAv.data_ptr())
         if L.graph is None:
-            nat.call('esr_mg_jacobi_pre', F64, L.nv, *csr, self.omega, r.data_ptr(),
+            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, self.omega, r.data_ptr(),
                      L.z.data_ptr(), L.t.data_ptr(), sp)
-            nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(),
+            nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(),
                      C.r.data_ptr(), sp)
             self._cycle(k + 1, C.r, C.zo)
             self._post(L, C, csr, r, out)
@@ -252,18 +274,18 @@ class MultigridPreconditioner:
         S = self.smoother
         if self.learned_pre:
             S.forward_into(L.graph, L.bufs, r, L.z)
-            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, None, *csr, r.data_ptr(),
+            nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, *csr, r.data_ptr(),
                      L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
         else:
-            nat.call('esr_mg_jacobi_pre', F64, L.nv, *csr, self.omega, r.data_ptr(),
+            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, self.omega, r.data_ptr(),
                      L.z.data_ptr(), L.t.data_ptr(), sp)
-        nat.call('esr_mg_restrict', F64, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
+        nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
                  sp)
         self._cycle(k + 1, C.r, C.zo)
         if self.learned_post:
-            nat.call('esr_mg_prolong', F64, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
+            nat.call('esr_mg_prolong', DT, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
                      L.z.data_ptr(), sp)
-            nat.call('esr_mg_residual', F64, L.nv, L.n_grid, None, *csr, r.data_ptr(),
+            nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, *csr, r.data_ptr(),
                      L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
             S.forward_into(L.graph, L.bufs, L.t, L.s)
             ops.ewise('add', L.z, L.s, out=out)
@@ -273,7 +295,7 @@ class MultigridPreconditioner:
     def _post(self, L, C, csr, r, out):
         """z += P zc (in place), then out = z + w (r - A z) / d."""
         sp = nat.stream_ptr()
-        nat.call('esr_mg_prolong', nat.F64, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
+        nat.call('esr_mg_prolong', self.dt, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
                  L.z.data_ptr(), sp)
-        nat.call('esr_mg_jacobi_post', nat.F64, L.n_grid, None, *csr, self.omega,
+        nat.call('esr_mg_jacobi_post', self.dt, L.n_grid, None, *csr, self.omega,
                  r.data_ptr(), L.z.data_ptr(), None, out.data_ptr(), sp)

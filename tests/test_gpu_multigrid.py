@@ -201,3 +201,39 @@ def test_multigrid_fcg_through_compile(gpu, smoother):
         assert info2['iters'] > 5 * info['iters'], (info, info2)
     x2 = eqn2.x.collect().numpy()
     assert np.linalg.norm(x - x2) <= 1e-6 * np.linalg.norm(x2)
+
+
+def test_float32_cycle_matches_oracle(gpu):
+    """The cycle in float32 (float32 copies of the float64 Galerkin
+    operators) against the oracle's float64 cycle: float32 rounding only."""
+    n = 64
+    A, fpos, csr, _ = _problem(n, 17, True)
+    M = mg.MultigridPreconditioner(coarsest=8, cycle_dtype=torch.float32).setup(*csr, n, fpos)
+    assert M.levels[0].z.dtype == torch.float32
+    ref = omg.hierarchy(A[fpos][:, fpos], n, fpos, coarsest=8)
+    r = np.random.default_rng(5).standard_normal(fpos.size)
+    z = M(torch.from_numpy(r).to(gpu))
+    assert z.dtype == torch.float64
+    want = omg.vcycle(ref, r, 0.8)
+    err = np.abs(z.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err <= 1e-5, err
+
+
+def test_float32_cycle_fcg_reaches_float64_tolerance(gpu):
+    """Mixed precision: flexible CG in float64 (dot products, iteration,
+    true residual) with a float32 cycle reaches 1e-10 on the true residual,
+    in about the steps of the float64 cycle."""
+    n = 128
+    its = {}
+    for cd in (torch.float64, torch.float32):
+        M = mg.MultigridPreconditioner(None, cycle_dtype=cd)
+        eqn, sol, sys_ = _compiled(n, 21, M)
+        info = sol.solve(rtol=1e-10, maxiter=500)
+        nv = sys_['Av'].size
+        A = sp.csr_matrix((sys_['Ae'], (sys_['row'], sys_['col'])), shape=(nv, nv)) + \
+            sp.diags(sys_['Av'])
+        x = eqn.x.collect().numpy()
+        assert np.linalg.norm(sys_['b'] - A @ x) <= 1.01e-10 * np.linalg.norm(sys_['b'])
+        assert info['fallback_at'] is None
+        its[cd] = info['iters']
+    assert its[torch.float32] <= its[torch.float64] + 3, its
