@@ -151,7 +151,7 @@ def poisson_config1(skip_cpu):
     return out
 
 
-def primitives(eqn, reps=20):
+def primitives(eqn, reps=20, with_cpu=False):
     """The plain Selector / Reducer kernels on the compiled SWE mesh: the
     cell -> edge gather of `gather_src` (esr_gather) and the edge -> cell CSR
     sum of `scatter` (esr_csr_reduce), each timed with HIP events over `reps`
@@ -190,11 +190,35 @@ def primitives(eqn, reps=20):
     bg = ne * (I + T) + nc * T
     br = ne * T + (nc + 1) * I + nc * T
     ba = nc * T
+    cpu = {}
+    if with_cpu:
+        # the oracle's operators on the host (torch CPU, all threads): t[idx]
+        # and zeros().scatter_reduce_(..., include_self=False), best of 3
+        from oracle import ops as oops
+        hsrc = src.cpu()
+        hidx = sel.idx.cpu().long()
+        hvals = edge_vals.cpu()
+        ncells = src.shape[0]
+        hrow = torch.repeat_interleave(torch.arange(ncells),
+                                       torch.diff(red.easier_rowptr.cpu().long()))
+
+        def host(fn):
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                fn()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            return best
+        cpu['gather'] = host(lambda: oops.selector(hsrc, hidx))
+        cpu['reduce'] = host(lambda: oops.reducer(hvals, hrow, ncells))
     return {
         'selector_gather': dict(edges=ne, seconds=t_g, bytes=bg, GBps=bg / t_g / 1e9,
-                                frac=bg / t_g / HBM_PEAK),
+                                frac=bg / t_g / HBM_PEAK,
+                                cpu_oracle_seconds=cpu.get('gather')),
         'reducer_csr_sum': dict(rows=nc, entries=ne, seconds=t_r, bytes=br,
-                                GBps=br / t_r / 1e9, frac=br / t_r / HBM_PEAK),
+                                GBps=br / t_r / 1e9, frac=br / t_r / HBM_PEAK,
+                                cpu_oracle_seconds=cpu.get('reduce')),
         'aggregate_sum': dict(n=nc, seconds=t_s, bytes=ba, GBps=ba / t_s / 1e9,
                               frac=ba / t_s / HBM_PEAK),
         'aggregate_norm': dict(n=nc, seconds=t_n, bytes=ba, GBps=ba / t_n / 1e9,
@@ -336,7 +360,23 @@ def batched_config3(reps=1):
     return out
 
 
-def mixed_config4(reps=1):
+def cpu_pcg_rate(sys_, iters=20):
+    """Seconds per Jacobi-PCG iteration of the oracle (torch CPU operators of
+    the reference's CPU backend, all host threads) on a system, from a
+    bounded run of `iters` iterations after one warm-up iteration."""
+    import torch
+    from oracle import solvers as osol
+    A = osol.Linsys(torch.from_numpy(sys_['Av']), torch.from_numpy(sys_['Ae']),
+                    torch.from_numpy(sys_['col']), torch.from_numpy(sys_['row']),
+                    sys_['Av'].shape[0])
+    b = torch.from_numpy(sys_['b'])
+    osol.cg(A, b, torch.zeros_like(b), M=lambda r: r / A.Av, rtol=0.0, maxiter=1)
+    t0 = time.perf_counter()
+    osol.cg(A, b, torch.zeros_like(b), M=lambda r: r / A.Av, rtol=0.0, maxiter=iters)
+    return (time.perf_counter() - t0) / (iters + 1)
+
+
+def mixed_config4(reps=1, skip_cpu=False):
     """Variable-coefficient Poisson on the 2048 x 2048 mesh (4,198,401
     vertices): float64 Jacobi-PCG to 1e-10 against the mixed-precision solver
     (float32 inner CG, float64 refinement) to 1e-10; error of the mixed
@@ -366,6 +406,13 @@ def mixed_config4(reps=1):
                bnorm=imx['bnorm'],
                rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64)))
     log(f'config 4: fp64 {t64:.3f} s ({i64["iters"]} it), mixed {tmx:.3f} s')
+    if not skip_cpu:
+        rate = cpu_pcg_rate(sys_)
+        out['cpu_fp64_s_per_iter'] = rate
+        out['cpu_fp64_solve_estimate_s'] = rate * i64['iters']
+        out['cpu_sample'] = ('21 Jacobi-PCG iterations of the oracle (torch CPU) on the '
+                             'same system; estimate = s/iter x the float64 iteration count')
+        log(f'config 4 CPU: {rate:.3f} s/iter')
     # the same problem with the multigrid preconditioner in flexible CG:
     # float64 cycle, and float32 cycle (float64 dots, iteration, residual)
     import torch
@@ -388,7 +435,7 @@ def mixed_config4(reps=1):
 
 
 def learned_config5(weights, seeds=(1,), reps=1,
-                    methods=('jacobi', 'mg', 'mg-learned')):
+                    methods=('jacobi', 'mg', 'mg-learned'), skip_cpu=False):
     """Learned-preconditioner flexible CG against Jacobi-PCG on unseen
     variable-coefficient problems on the 2048 x 2048 mesh, all to 1e-8:
     'mg-learned' is flexible CG preconditioned by a multigrid V-cycle whose
@@ -396,10 +443,18 @@ def learned_config5(weights, seeds=(1,), reps=1,
     smoothing (the method without the network)."""
     from easier_amd.numeric.preconditioner import learned as lp
     from easier_amd.workloads import learned as wl
+    from easier_amd.workloads import poisson as wp
     pre = lp.load(weights)
-    return [wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000,
-                                 methods=methods)
-            for s in seeds]
+    out = []
+    for s in seeds:
+        res = wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000,
+                                   methods=methods)
+        if not skip_cpu and 'jacobi' in res:
+            rate = cpu_pcg_rate(wp.varcoef_system(2048, s)[1])
+            res['cpu_jacobi_s_per_iter'] = rate
+            res['cpu_jacobi_solve_estimate_s'] = rate * res['jacobi']['iters']
+        out.append(res)
+    return out
 
 
 def copy_calibration(mib):
@@ -525,7 +580,7 @@ def main():
         }
         log(f'dominant kernel esr_fused_{k}: {ktimes[k] * 1e6:.1f} us, '
             f'{achieved / 1e9:.0f} GB/s ({achieved / HBM_PEAK:.1%} of 8 TB/s)')
-    result['primitives'] = primitives(eqn)
+    result['primitives'] = primitives(eqn, with_cpu=not args.skip_cpu)
     if not args.skip_cpu:
         log(f'CPU baseline: {args.cpu_steps} oracle steps on the host')
         result['cpu_baseline'] = cpu_baseline_swe(mesh, geo, meta, dtype, args.cpu_steps)
@@ -544,9 +599,9 @@ def main():
     if '3' in extra:
         result['batched_config3'] = batched_config3()
     if '4' in extra:
-        result['mixed_config4'] = mixed_config4()
+        result['mixed_config4'] = mixed_config4(skip_cpu=args.skip_cpu)
     if '5' in extra:
-        result['learned_config5'] = learned_config5(args.weights)
+        result['learned_config5'] = learned_config5(args.weights, skip_cpu=args.skip_cpu)
     print(json.dumps(result), flush=True)
 
 

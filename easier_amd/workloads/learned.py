@@ -432,7 +432,8 @@ def evaluate_compiled(pre: Optional[LearnedPreconditioner], n, seed, rtol=1e-8, 
 
 
 def _log(msg):
-    print(f'[learned {time.strftime("%H:%M:%S")}] {msg}', flush=True)
+    import sys
+    print(f'[learned {time.strftime("%H:%M:%S")}] {msg}', file=sys.stderr, flush=True)
 
 
 def main(argv=None):
