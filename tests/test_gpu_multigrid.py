@@ -237,3 +237,37 @@ def test_float32_cycle_fcg_reaches_float64_tolerance(gpu):
         assert info['fallback_at'] is None
         its[cd] = info['iters']
     assert its[torch.float32] <= its[torch.float64] + 3, its
+
+
+def test_multigrid_on_jittered_mesh(gpu):
+    """Interior vertices jittered by up to 0.2 h (the batched family's
+    meshes): grid positions are recovered from the coordinates, P is the
+    index-based interpolation (no longer exact P1 nesting), and flexible CG
+    still converges in a few dozen steps on the true residual."""
+    n = 128
+    mesh, sys_, kap = wp.varcoef_system(n, 5, jitter=0.2)
+    logk = wp.gaussian_random_field(n, 5).reshape(-1)
+    eqn = wp.PoissonP1(sys_)
+    sol = FlexibleCG(eqn.A, eqn.b, eqn.x, mg.MultigridPreconditioner(None),
+                     dict(points=mesh['points'], logk=logk, n_grid=n))
+    esr.compile([eqn, sol], 'hip')
+    info = sol.solve(rtol=1e-8, maxiter=500)
+    nv = sys_['Av'].size
+    A = sp.csr_matrix((sys_['Ae'], (sys_['row'], sys_['col'])), shape=(nv, nv)) + \
+        sp.diags(sys_['Av'])
+    x = eqn.x.collect().numpy()
+    assert np.linalg.norm(sys_['b'] - A @ x) <= 1.01e-8 * np.linalg.norm(sys_['b'])
+    assert info['fallback_at'] is None and info['iters'] <= 40, info
+
+
+def test_setup_rejects_meshes_it_cannot_coarsen(gpu):
+    n = 100                      # 100 -> 50 -> 25: an odd grid of 25 <= 32, accepted
+    A, fpos, csr, _ = _problem(n, 1, False)
+    M = mg.MultigridPreconditioner(coarsest=16).setup(*csr, n, fpos)
+    assert [L.n_grid for L in M.levels] == [100, 50, 25]
+    n = 2 * 37                   # 74 -> 37: odd and above 32
+    A, fpos, csr, _ = _problem(n, 1, False)
+    with pytest.raises(ValueError, match='coarsest'):
+        mg.MultigridPreconditioner(coarsest=16).setup(*csr, n, fpos)
+    with pytest.raises(ValueError, match='grid'):
+        mg.MultigridPreconditioner().setup(*csr, n + 1, fpos)
