@@ -222,24 +222,25 @@ __global__ __launch_bounds__(kMB) void k_mg_prolong(int64_t nf, const int32_t *_
 }
 
 // Damped-Jacobi pre-smoothing from z = 0 and the residual it leaves:
-// z = (w r) / d; t = r - A z.
+// z = wd r; t = r - A z, with wd = w / d per row (precomputed, so the sweep
+// multiplies instead of dividing at every neighbour).
 template <typename T>
 __global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ col,
                                                       const T *__restrict__ Ae,
-                                                      const T *__restrict__ Av, T omega,
+                                                      const T *__restrict__ Av,
+                                                      const T *__restrict__ wd,
                                                       const T *__restrict__ r, T *__restrict__ z,
                                                       T *__restrict__ t) {
   const int64_t i = mg_tid();
   if (i >= n) return;
-  const T az = linsys_row<T>(i, rowptr, col, Ae, Av,
-                             [&](int64_t j) { return (omega * r[j]) / Av[j]; });
-  z[i] = (omega * r[i]) / Av[i];
+  const T az = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return wd[j] * r[j]; });
+  z[i] = wd[i] * r[i];
   t[i] = r[i] - az;
 }
 
 // Coarse-grid correction and damped-Jacobi post-smoothing:
-// y = z + P zc; out = y + (w (r - A y)) / d. With zc null, y = z (the
+// y = z + P zc; out = y + wd (r - A y), wd = w / d. With zc null, y = z (the
 // correction already added by k_mg_prolong): the fused form evaluates P at
 // every neighbour (a rowptr -> col -> parent -> zc load chain), which runs
 // about 3x slower on the finest level than the two-pass form.
@@ -248,7 +249,8 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_
                                                        const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const T *__restrict__ Ae,
-                                                       const T *__restrict__ Av, T omega,
+                                                       const T *__restrict__ Av,
+                                                       const T *__restrict__ wd,
                                                        const T *__restrict__ r,
                                                        const T *__restrict__ z,
                                                        const T *__restrict__ zc,
@@ -259,10 +261,10 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_
     auto y = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
     const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, y);
     const T yi = y(i);
-    out[i] = yi + (omega * (r[i] - ay)) / Av[i];
+    out[i] = yi + wd[i] * (r[i] - ay);
   } else {
     const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return z[j]; });
-    out[i] = z[i] + (omega * (r[i] - ay)) / Av[i];
+    out[i] = z[i] + wd[i] * (r[i] - ay);
   }
 }
 
@@ -399,13 +401,13 @@ ESR_API int esr_mg_prolong(int dtype, int64_t nf, const int32_t *par, const void
 }
 
 ESR_API int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr, const int32_t *col,
-                              const void *Ae, const void *Av, double omega, const void *r,
+                              const void *Ae, const void *Av, const void *wd, const void *r,
                               void *z, void *t, void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(n >= 1, "esr_mg_jacobi_pre: n must be >= 1");
-  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && t, "esr_mg_jacobi_pre: null pointer");
+  ESR_REQUIRE(rowptr && col && Ae && Av && wd && r && z && t, "esr_mg_jacobi_pre: null pointer");
   MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_pre<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
-                               n, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
+                               n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)wd,
                                (const T *)r, (T *)z, (T *)t)));
   ESR_LAUNCHED();
   return ESR_OK;
@@ -413,17 +415,17 @@ ESR_API int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr, const
 }
 
 ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *par, const int32_t *rowptr,
-                               const int32_t *col, const void *Ae, const void *Av, double omega,
+                               const int32_t *col, const void *Ae, const void *Av, const void *wd,
                                const void *r, const void *z, const void *zc, void *out,
                                void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_jacobi_post: fine grid n=%lld is not even", (long long)nf);
-  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && out && (par || !zc),
+  ESR_REQUIRE(rowptr && col && Ae && Av && wd && r && z && out && (par || !zc),
               "esr_mg_jacobi_post: null pointer");
   ESR_REQUIRE(out != z, "esr_mg_jacobi_post: out must not alias z");
   const int64_t nv = (nf + 1) * (nf + 1);
   MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
-                               nf, par, rowptr, col, (const T *)Ae, (const T *)Av, (T)omega,
+                               nf, par, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)wd,
                                (const T *)r, (const T *)z, (const T *)zc, (T *)out)));
   ESR_LAUNCHED();
   return ESR_OK;

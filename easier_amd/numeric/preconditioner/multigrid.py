@@ -18,7 +18,8 @@ cycle solves with the dense inverse.
 
 Cycle, per level (r in, z out):
   Jacobi:  z0 = w r/d; t = r - A z0; zc = V(P^T t);
-           y = z0 + P zc; z = y + w (r - A y)/d           (4 kernels)
+           y = z0 + P zc; z = y + w (r - A y)/d           (4 kernels; w/d
+           precomputed per level)
   learned: z0 = S(r); t = r - A z0; zc = V(P^T t);
            then the Jacobi post-smoothing above (default), or
            y = z0 + P zc; t = r - A y; z = y + S(t)
@@ -45,10 +46,8 @@ nat.register_signatures({
     'esr_mg_parents': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp]),
     'esr_mg_restrict': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_mg_prolong': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
-    'esr_mg_jacobi_pre': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl,
-                                   _c_vp, _c_vp, _c_vp, _c_vp]),
-    'esr_mg_jacobi_post': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
-                                    _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_jacobi_pre': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 9),
+    'esr_mg_jacobi_post': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 11),
     'esr_mg_residual': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 11),
     'esr_mg_dense': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
 })
@@ -192,6 +191,9 @@ class MultigridPreconditioner:
             # the operator the cycle reads (a float32 copy in a float32 cycle)
             L.cAe = L.Ae if cd == torch.float64 else L.Ae.to(cd)
             L.cAv = L.Av if cd == torch.float64 else L.Av.to(cd)
+            L.wd = ops.ewise('div', self.omega, L.Av)         # w / d, float64
+            if cd != torch.float64:
+                L.wd = L.wd.to(cd)
             L.z = torch.zeros(L.nv, dtype=cd, device=dev)
             L.t = torch.zeros(L.nv, dtype=cd, device=dev)
             if k > 0:
@@ -264,7 +266,7 @@ class MultigridPreconditioner:
         P = lambda t: None if t is None else t.data_ptr()
         csr = (L.rowptr.data_ptr(), L.col.data_ptr(), L.cAe.data_ptr(), L.cAv.data_ptr())
         if L.graph is None:
-            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, self.omega, r.data_ptr(),
+            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, L.wd.data_ptr(), r.data_ptr(),
                      L.z.data_ptr(), L.t.data_ptr(), sp)
             nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(),
                      C.r.data_ptr(), sp)
@@ -277,7 +279,7 @@ class MultigridPreconditioner:
             nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, *csr, r.data_ptr(),
                      L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
         else:
-            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, self.omega, r.data_ptr(),
+            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, L.wd.data_ptr(), r.data_ptr(),
                      L.z.data_ptr(), L.t.data_ptr(), sp)
         nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
                  sp)
@@ -297,5 +299,5 @@ class MultigridPreconditioner:
         sp = nat.stream_ptr()
         nat.call('esr_mg_prolong', self.dt, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
                  L.z.data_ptr(), sp)
-        nat.call('esr_mg_jacobi_post', self.dt, L.n_grid, None, *csr, self.omega,
+        nat.call('esr_mg_jacobi_post', self.dt, L.n_grid, None, *csr, L.wd.data_ptr(),
                  r.data_ptr(), L.z.data_ptr(), None, out.data_ptr(), sp)

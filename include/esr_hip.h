@@ -411,9 +411,9 @@ int esr_fcg_read(const void *state, double *scalars, int64_t *ints, void *stream
  *   parents, -1 = boundary/none, second = -2 for a single parent), read by
  *   the kernels that apply P instead of dividing grid positions.
  * esr_mg_restrict: rc = P^T t. esr_mg_prolong: z += P zc.
- * esr_mg_jacobi_pre: z = w r / d, t = r - A z.
+ * esr_mg_jacobi_pre: z = wd r, t = r - A z (wd = w / d per row).
  * esr_mg_jacobi_post: y = z + P zc (y = z when zc is NULL),
- *   out = y + w (r - A y) / d.
+ *   out = y + wd (r - A y).
  * esr_mg_residual: t = r - A y, y = z (zc NULL) or y = z + P zc (stored).
  * esr_mg_dense: z = Ainv r (coarsest grid, Ainv row-major float64).
  * dtype ESR_F64 or ESR_F32 (vectors and the operator's Ae/Av).           */
@@ -433,10 +433,10 @@ int esr_mg_prolong(int dtype, int64_t nf, const int32_t *par, const void *zc,
                    void *z, void *stream);
 int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr,
                       const int32_t *col, const void *Ae, const void *Av,
-                      double omega, const void *r, void *z, void *t, void *stream);
+                      const void *wd, const void *r, void *z, void *t, void *stream);
 int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *par,
                        const int32_t *rowptr, const int32_t *col,
-                       const void *Ae, const void *Av, double omega,
+                       const void *Ae, const void *Av, const void *wd,
                        const void *r, const void *z, const void *zc, void *out,
                        void *stream);
 int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *par,
