@@ -13,12 +13,13 @@ from .core.module import (
     hdf5, full, zeros, ones, full_like, zeros_like, ones_like, arange,
 )
 from .core.jit import init, compile
+from .core.cache import dump
 from .core.utils import logger, EasierJitException
 from . import numeric
 
 __all__ = [
     'Tensor', 'Module', 'Selector', 'Reducer', 'sum', 'prod', 'norm', 'max',
     'min', 'hdf5', 'full', 'zeros', 'ones', 'full_like', 'zeros_like',
-    'ones_like', 'arange', 'init', 'compile', 'logger', 'EasierJitException',
+    'ones_like', 'arange', 'init', 'compile', 'dump', 'logger', 'EasierJitException',
     'numeric',
 ]

@@ -349,7 +349,10 @@ def _validate_idx(name, idx: np.ndarray, upper: Optional[int], kind: str):
 
 
 def compile_program(top_modules: List[_M.Module], reorder: str,
-                    device: torch.device) -> CompiledProgram:
+                    device: torch.device, cache: Optional[dict] = None) -> CompiledProgram:
+    """cache: a loaded compile cache (core/cache.py); its space orders
+    replace the renumbering when it matches this program."""
+    from . import cache as _cache
     prog = CompiledProgram()
     prog.device = device
     objs = get_easier_objects(top_modules)
@@ -404,6 +407,13 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
         vin = sa.root(sa.sel_vars[s][0])
         _validate_idx(s.easier_hint_name, idx, space_len.get(vin), 'selector')
 
+    prog.fingerprint = _cache.fingerprint(modules, graphs, objs, sel_idx, red_idx, reorder,
+                                          nat.lib().esr_abi_version())
+    prog.space_name = _cache.space_names(sa, objs)
+    cached = _cache.matching_orders(cache, prog.fingerprint, prog.space_name, space_len)
+    if cached is not None:
+        logger.info(f'compile cache: reusing the order of {len(cached)} spaces')
+
     # ---- renumbering ------------------------------------------------------
     red_in = {r: sa.root(sa.red_vars[r][0]) for r in reducers}
     red_out = {r: sa.root(sa.red_vars[r][1]) for r in reducers}
@@ -412,8 +422,11 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
     reducer_inputs = set(red_in.values())
 
     order: Dict[int, Optional[np.ndarray]] = {}
+    kind: Dict[int, str] = {}
     # node spaces: RCM over (reducer target, selector source) pairs
-    if reorder == 'rcm':
+    if cached is not None:
+        order = {root: o for root, (k, o) in cached.items() if k == 'rcm'}
+    elif reorder == 'rcm':
         pairs: Dict[int, list] = {}
         for r in reducers:
             for s in selectors:
@@ -434,6 +447,8 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
                          f'{bandwidth(a, b)} -> {bandwidth(inv[a], inv[b])}')
     elif reorder != 'none':
         raise EasierJitException(f"reorder must be 'rcm' or 'none', got {reorder!r}")
+    for root in order:
+        kind[root] = 'rcm'
 
     def inv_of(space):
         o = order.get(space)
@@ -445,14 +460,23 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
         if e in order or e in prog.bound_reducer:
             continue
         prog.bound_reducer[e] = r
+        if cached is not None:
+            if e in cached:
+                order[e], kind[e] = cached[e][1], 'edge'
+            continue
         inv = inv_of(red_out[r])
         key = red_idx[r] if inv is None else inv[red_idx[r]]
         _, perm = csr_from_idx(key, r.n)
         if not np.array_equal(perm, np.arange(perm.shape[0])):
-            order[e] = perm
+            order[e], kind[e] = perm, 'edge'
     for s in selectors:
         e = sel_out[s]
         if e in order or e in reducer_inputs or e in prog.bound_reducer:
+            continue
+        if cached is not None:
+            if e in cached:
+                order[e], kind[e] = cached[e][1], 'edge'
+            prog.bound_reducer[e] = None
             continue
         # a Selector never applied in a traced forward has no known input
         # length: bound its index range by its own maximum
@@ -461,9 +485,10 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
         key = sel_idx[s] if inv is None else inv[sel_idx[s]]
         _, perm = csr_from_idx(key, n_in)
         if not np.array_equal(perm, np.arange(perm.shape[0])):
-            order[e] = perm
+            order[e], kind[e] = perm, 'edge'
         prog.bound_reducer[e] = None
     prog.space_order = order
+    prog.space_kind = kind
     prog.space_len = space_len
 
     # ---- load data in the new order ---------------------------------------

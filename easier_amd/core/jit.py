@@ -74,11 +74,14 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
       of it (Tensor.collect() returns the user's order).
     - fuse: lower supported forwards into fused kernels (hipRTC); otherwise
       every node runs as its own primitive kernel.
+    - load_dir: a directory written by esr.dump(); its renumbering is reused
+      when it matches this program (core/cache.py), else compiled afresh.
     """
     top_modules = _validate_compile_args(modules, backend, partition_mode)
+    cache = None
     if load_dir is not None:
-        raise EasierJitException(
-            'load_dir (compile cache) is not supported by the hip backend')
+        from .cache import load as load_cache
+        cache = load_cache(load_dir)
     if not torch.cuda.is_available():
         raise RuntimeError("backend='hip' needs a visible AMD GPU "
                            "(torch.cuda.is_available() is False)")
@@ -86,7 +89,7 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
     device = torch.device('cuda', torch.cuda.current_device())
     logger.info(f"EASIER just-in-time compilation has started, "
                 f"backend={backend}, device={device}")
-    prog = compile_program(top_modules, reorder, device)
+    prog = compile_program(top_modules, reorder, device, cache)
     from .engine import HipEngine
     fused = None
     if fuse:

@@ -209,6 +209,30 @@ def test_two_rows_per_thread_kernels_are_bit_identical(gpu, dtype):
         assert torch.equal(outs[1][0], ref.h)
 
 
+def test_compile_cache_round_trip(gpu, tmp_path):
+    """esr.dump() then esr.compile(load_dir=...) on a fresh program: the same
+    renumbering, the same SWE steps bit for bit."""
+    mesh = wmesh.tri2(12)
+    geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
+    dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], 9.81)
+    states = []
+    for load in (None, str(tmp_path)):
+        eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=9.81)
+        esr.compile([eqn], 'hip', load_dir=load)
+        if load is None:
+            esr.dump([eqn], str(tmp_path))
+        for _ in range(5):
+            eqn()
+        states.append([eqn.h.collect(), eqn.uh.collect(), eqn.vh.collect(),
+                       {eqn.easier_program.space_name[k]: v
+                        for k, v in eqn.easier_program.space_order.items()}])
+    for a, b in zip(states[0][:3], states[1][:3]):
+        assert torch.equal(a, b)
+    assert states[0][3].keys() == states[1][3].keys()
+    for k in states[0][3]:
+        assert np.array_equal(states[0][3][k], states[1][3][k])
+
+
 def test_fused_cg_graph_replays_are_deterministic(gpu):
     eqn = Poisson5(40, seed=3)
     sol = CG(eqn.A, eqn.b, eqn.x, Jacobi(eqn.Ac))
