@@ -82,9 +82,17 @@ class Unsupported(Exception):
 #                  the 2048 x 2048 SWE step: 1.38 vs 1.18 ms (half the threads
 #                  leave fewer gathers in flight than the wider row accesses
 #                  save), so the default is 1.
+#   pack_intermediates - the values one kernel materialises for later kernels
+#                  share one buffer, interleaved by row (value j of row r at
+#                  r * width + j): one array stream instead of `width` for
+#                  their writer and readers (rows_per_thread 1 only).
+#                  Measured on the 2048 x 2048 SWE step: 1.41 vs 1.18 ms (a
+#                  load instruction then spans width x the rows of one
+#                  stream and uses 1/width of each line it touches), so the
+#                  default is off.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64,
-           'rows_per_thread': 1}
+           'rows_per_thread': 1, 'pack_intermediates': False}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -171,6 +179,7 @@ class V:
         self.users: List['V'] = []
         self.mat = False          # written to a buffer by its kernel
         self.buf: Optional[torch.Tensor] = None
+        self.pack = None          # (width, column) in a row-interleaved buffer
         for a in self.args:
             a.users.append(self)
 
@@ -574,6 +583,20 @@ class _Lowering:
             if need:
                 n = self.prog.space_len[v.space]
                 v.buf = torch.empty(n, dtype=p.dtype, device=self.device)
+        if OPTIONS['pack_intermediates'] and OPTIONS['rows_per_thread'] == 1:
+            # the values one kernel materialises share one row-interleaved
+            # buffer: row r of value j at r * width + j
+            groups: Dict[tuple, list] = {}
+            for v in p.values:
+                if v.mat and v.kind != 'leaf':
+                    groups.setdefault((v.stage, v.space), []).append(v)
+            for (stage, space), vs in groups.items():
+                if len(vs) < 2:
+                    continue
+                n = self.prog.space_len[space]
+                buf = torch.empty(n * len(vs), dtype=p.dtype, device=self.device)
+                for j, v in enumerate(vs):
+                    v.buf, v.pack = buf, (len(vs), j)
 
     def build(self):
         p = self.p
@@ -652,10 +675,16 @@ class _Lowering:
             assert v.mat and v.buf is not None
             return self._param(kern, v.buf, ct)
 
-        def row_load(buf):
+        def elem(v, row):
+            """Index of v's element at `row` in its buffer."""
+            if v is not None and v.pack is not None:
+                return f'({row}) * {v.pack[0]} + {v.pack[1]}'
+            return row
+
+        def row_load(buf, v=None):
             """buf[row] in the current row context."""
             if cx.comp is None:
-                return var(cx.rows, f'{buf}[{cx.R}]')
+                return var(cx.rows, f'{buf}[{elem(v, cx.R)}]')
             if buf not in vloads:
                 vloads[buf] = f'w{len(vloads)}'
                 prologue.append(f'  const {vt} {vloads[buf]} = esr_ld2({buf}, r0, a.n);')
@@ -715,12 +744,12 @@ class _Lowering:
                     (v.mat and v.space != kern.space):
                 if v.space != kern.space:
                     raise Unsupported('row read from another space')
-                s = row_load(buf_of(v))
+                s = row_load(buf_of(v), v)
             elif v.kind == 'gather':
                 sel = v.obj
                 src = v.args[0]
                 idx = self._param(kern, sel.idx, 'int')
-                s = var(cx.rows, f'{buf_of(src)}[{idx}[{cx.R}]]')
+                s = var(cx.rows, f'{buf_of(src)}[{elem(src, f"{idx}[{cx.R}]")}]')
             elif v.kind == 'reduce':
                 emit_group(v)
                 s = cx.memo[v.id]
@@ -811,7 +840,7 @@ class _Lowering:
                         s = emit_row(src) if src.kind != 'leaf' else \
                             emit_row_leaf(src)
                     else:
-                        s = var(body, f'{buf_of(src)}[{gather_index(x.obj, src)}]')
+                        s = var(body, f'{buf_of(src)}[{elem(src, gather_index(x.obj, src))}]')
                 elif x.kind == 'ew':
                     a = [emit_edge(y) for y in x.args]
                     s = var(body, ew_expr(x, a), 'bool' if is_pred(x) else None)
@@ -923,30 +952,32 @@ class _Lowering:
                 if rt.kind == 'store':
                     val = rt.args[0]
                     if val.mat and val.stage != kern.stage:
-                        s = row_load(buf_of(val))
+                        s = row_load(buf_of(val), val)
                     else:
                         s = emit_row(val)
-                    stores.append((self._param(kern, rt.obj.data, ct, const=False), s))
+                    stores.append((self._param(kern, rt.obj.data, ct, const=False), s,
+                                   None))
                 else:
                     s = emit_row(rt)
-                    stores.append((self._param(kern, rt.buf, ct, const=False), s))
+                    stores.append((self._param(kern, rt.buf, ct, const=False), s, rt))
             ctx_stores.append(stores)
             body_rows.extend(cx.rows)
         rows = prologue + body_rows
         nt_st = OPTIONS['nt_stores']
         if rpt == 1:
             if nt_st:
-                st = [f'  __builtin_nontemporal_store({s}, &{d}[r]);' for d, s in ctx_stores[0]]
+                st = [f'  __builtin_nontemporal_store({s}, &{d}[{elem(v, "r")}]);'
+                      for d, s, v in ctx_stores[0]]
             else:
-                st = [f'  {d}[r] = {s};' for d, s in ctx_stores[0]]
+                st = [f'  {d}[{elem(v, "r")}] = {s};' for d, s, v in ctx_stores[0]]
         else:
             fn = 'esr_st2nt' if nt_st else 'esr_st2'
             st = [f'  {fn}({d}, r0, a.n, {sa}, {sb});'
-                  for (d, sa), (_, sb) in zip(ctx_stores[0], ctx_stores[1])]
+                  for (d, sa, _), (_, sb, _) in zip(ctx_stores[0], ctx_stores[1])]
             # the paired accesses are 2-element vectors: their arrays must be
             # aligned to the vector (else this kernel keeps one row per thread)
             vec = 2 * self.p.dtype.itemsize
-            names = set(vloads) | {d for d, _ in ctx_stores[0]}
+            names = set(vloads) | {d for d, _, _ in ctx_stores[0]}
             for cname, _, t in kern.params:
                 if cname in names and t.data_ptr() % vec:
                     raise _RowsPerThread()

@@ -88,7 +88,9 @@ def test_sparse_row_bitmap_finds_every_row():
     dict(stage_edges=False, nt_loads=False, nt_stores=True, block=256, xcd_remap=False,
          xcd_group=64, rows_per_thread=2),
     dict(stage_edges=False, nt_loads=False, nt_stores=False, block=128, xcd_remap=False,
-         xcd_group=0, rows_per_thread=2)])
+         xcd_group=0, rows_per_thread=2),
+    dict(stage_edges=False, nt_loads=False, nt_stores=True, block=256, xcd_remap=False,
+         xcd_group=64, rows_per_thread=1, pack_intermediates=True)])
 def test_code_generation_options(opts):
     mesh = wmesh.tri2(7)
     geo = oswe.geometry(mesh)

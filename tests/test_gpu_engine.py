@@ -157,14 +157,16 @@ def test_swe_steps_match_oracle(gpu, dtype, gen, g, fuse):
             assert torch.equal(got, want), k
 
 
+@pytest.mark.parametrize('option', [('rows_per_thread', 2), ('pack_intermediates', True)])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
-def test_two_rows_per_thread_kernels_are_bit_identical(gpu, dtype):
-    """The fused kernels evaluating two rows per thread (vector row
-    accesses) give the same bits as one row per thread: the SWE step (even
-    cell count) and a program on an odd number of rows (the last thread's
-    second row is past the end)."""
+def test_codegen_options_are_bit_identical(gpu, dtype, option):
+    """Fused kernels generated with two rows per thread (vector row
+    accesses) or with row-interleaved intermediate buffers give the same bits
+    as the defaults: the SWE step (even cell count) and a program on an odd
+    number of rows (the last thread's second row is past the end)."""
     from easier_amd.core import fuser
-    print(f'two rows per thread {dtype}', flush=True)
+    key, val = option
+    print(f'codegen {key}={val} {dtype}', flush=True)
     mesh = wmesh.tri2(10)
     geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
     dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], 9.81)
@@ -188,12 +190,13 @@ def test_two_rows_per_thread_kernels_are_bit_identical(gpu, dtype):
     outs = []
     saved = dict(fuser.OPTIONS)
     try:
-        for rpt in (1, 2):
-            fuser.OPTIONS['rows_per_thread'] = rpt
+        for v in (saved[key], val):
+            fuser.OPTIONS[key] = v
             eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=9.81, dtype=dtype)
             odd = Odd()
             esr.compile([eqn, odd], 'hip')
-            assert all(k.rpt == rpt for k in eqn.easier_engine.kernels)
+            if key == 'rows_per_thread':
+                assert all(k.rpt == v for k in eqn.easier_engine.kernels)
             for _ in range(5):
                 eqn()
                 odd()
