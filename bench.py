@@ -560,6 +560,7 @@ def main():
             'step_bytes_algorithmic': step_bytes,
             'step_bytes_formula': swe_bytes(nc, ne, nbc, T)[1],
             'step_hbm_fraction': step_bytes / (wall / args.steps) / HBM_PEAK,
+            'edges_per_s': ne * args.steps / wall,
             'state_finite': finite,
         },
     }
