@@ -90,9 +90,15 @@ class Unsupported(Exception):
 #                  load instruction then spans width x the rows of one
 #                  stream and uses 1/width of each line it touches), so the
 #                  default is off.
+#   waves_per_eu - N > 0: ask the register allocator for N waves per SIMD
+#                  (amdgpu_waves_per_eu); 0 leaves occupancy to the compiler.
+#                  On the SWE 2048^2 float64 step the four stage kernels run
+#                  8/7/7/6 waves by default; N = 8 left the first three at
+#                  256/318/318 us and made the fourth spill 44 bytes per lane
+#                  (292 -> 417 us, step 1.18 -> 1.31 ms), so the default is 0.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64,
-           'rows_per_thread': 1, 'pack_intermediates': False}
+           'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -1005,9 +1011,11 @@ class _Lowering:
             head = (f'  const i64 r = {tile} * {kern.block} + threadIdx.x;\n'
                     '  if (r >= a.n) return;\n')
             tail = '\n'.join(st)
+        wpe = (f' __attribute__((amdgpu_waves_per_eu({int(OPTIONS["waves_per_eu"])})))'
+               if OPTIONS['waves_per_eu'] else '')
         kern.src = (
             f'struct Args_{kern.name} {{\n{params}  i64 n;\n}};\n'
-            f'extern "C" __global__ __launch_bounds__({kern.block}) void {kern.name}'
+            f'extern "C" __global__ __launch_bounds__({kern.block}){wpe} void {kern.name}'
             f'(const Args_{kern.name} a) {{\n' + head + '\n'.join(rows) + '\n' +
             tail + '\n}\n')
 
