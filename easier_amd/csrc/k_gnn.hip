@@ -10,13 +10,19 @@
 // q() = round to bfloat16. The per-vertex products (h Wp, h Wq, [h,a] Wu) run
 // on the matrix cores: v_mfma_f32_16x16x32_bf16, bfloat16 operands, float32
 // accumulation. A wavefront owns 16 vertices; lane l holds vertex l%16 and
-// features 8*(l/16)..+7, which is the MFMA A-operand layout, so message
-// aggregation feeds the update MLP without a shuffle. MFMA results come back
-// row-per-lane through LDS.
+// eight of its features. The products are computed transposed (W^T x^T):
+// x in this layout is the B operand, and the result returns in the same
+// layout (with a fixed feature permutation, the slot order below), so
+// message aggregation feeds the update MLP and each product feeds the next
+// without a shuffle, an LDS round trip or a barrier. The edge term of every
+// message, e_ij We + bm + P_i, is one float32 matrix-core product per edge
+// slot as well ([We; bm]^T [e_ij; 1] accumulated onto P_i^T).
 //
 // The backward pass (training) is hand-written float32 code, one thread per
 // vertex, with weight gradients reduced deterministically (block partials in
 // block order, then a fixed-order sum).
+#include <type_traits>
+
 #include "esr_common.h"
 
 namespace esr {
@@ -24,8 +30,8 @@ namespace {
 
 constexpr int H = 32;
 constexpr int kB = 256;   // backward / elementwise kernels
-// forward kernels: one wavefront per workgroup, so the LDS round trips of
-// the MFMA layout changes (c_to_rows) synchronise one wave, not four
+// forward kernels: one wavefront (16 vertices) per workgroup (measured
+// faster than 2 or 4 wavefronts, or several vertex groups per wavefront)
 constexpr int kFB = 64;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -44,43 +50,51 @@ struct Off {
   __host__ __device__ int b_out() const { return w_out() + H; }
   __host__ __device__ int total() const { return b_out() + 1; }
 };
-// packed bfloat16 matrices per layer, transposed (output-major):
+// packed bfloat16 matrices per layer, transposed (output-major), the input
+// (K) columns of every H-wide block in slot order (see slot_feature):
 // WpT [H][H], WqT [H][H], WuT [H][2H]
 constexpr int kPWp = 0, kPWq = H * H, kPWu = 2 * H * H, kPLayer = 4 * H * H;
 
 __device__ __forceinline__ float relu(float x) { return x > 0.0f ? x : 0.0f; }
 __device__ __forceinline__ float q16(float x) { return (float)(__bf16)x; }
 
-// C[16][32] (two 16x16 blocks) += A[16][32*KB] * W[32*KB][32], W given
-// transposed (WT[n][k], row length ldk). Lane layout: A chunk kb holds
-// A[l%16][32kb + 8(l/16) .. +7]; c[nb][v] = C[4(l/16)+v][16nb + l%16].
+// Feature slots. Lane l of a wavefront holds vertex l%16 and, in its eight
+// slots f, the logical features F(g, f) = 16 (f/4) + 4 g + f%4 with
+// g = l/16: the layout in which the matrix cores return W^T x^T. Per-vertex
+// activations (h, Q, the saved aggregates a) are stored in slot order,
+// position 8 g + f, so a lane moves its features with one 16-byte access.
+__host__ __device__ constexpr int slot_feature(int p) {
+  return 16 * ((p & 7) >> 2) + 4 * (p >> 3) + (p & 3);
+}
+__host__ __device__ constexpr int feature_slot(int F) {
+  return 8 * ((F & 15) >> 2) + 4 * (F >> 4) + (F & 3);
+}
+static_assert(feature_slot(slot_feature(13)) == 13 && slot_feature(feature_slot(21)) == 21,
+              "slot maps are inverse");
+// packed column of logical input k < 2H of the update MLP
+__host__ __device__ constexpr int wu_col(int k) { return (k / H) * H + feature_slot(k % H); }
+
+// out (slot layout) = x W for the 16 vertices of the wavefront, computed as
+// C^T = W^T x^T: the packed weights WT[m][32 kb + p] = W[32 kb + F(p)][m] are
+// the A operand (row m = output feature), x in slot layout is the B operand
+// (lane: vertex l%16, K positions 8(l/16)..+7), and C^T comes back with
+// lane l holding output features 16 nb + 4(l/16) + v of vertex l%16, which
+// is slot 4 nb + v. No LDS round trip, no barrier.
 template <int KB>
-__device__ __forceinline__ void mfma_rows(const bf16x8 (&a)[KB], const __bf16 *__restrict__ WT,
-                                          int ldk, floatx4 (&c)[2]) {
+__device__ __forceinline__ void mfma_slots(const bf16x8 (&x)[KB], const __bf16 *__restrict__ WT,
+                                           int ldk, float (&out)[8]) {
   const int l = threadIdx.x & 63, g = l >> 4, j = l & 15;
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
-    c[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(WT + (16 * nb + j) * ldk + 32 * kb + 8 * g);
-      c[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], b, c[nb], 0, 0, 0);
+      const bf16x8 w = *reinterpret_cast<const bf16x8 *>(WT + (16 * nb + j) * ldk + 32 * kb + 8 * g);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[kb], c, 0, 0, 0);
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[4 * nb + v] = c[v];
   }
-}
-
-// MFMA C layout -> row layout (lane l: row l%16, columns 8(l/16)..+7).
-// Every wave of the block must call it (block barriers).
-__device__ __forceinline__ void c_to_rows(const floatx4 (&c)[2], float (*T)[33], float (&out)[8]) {
-  const int l = threadIdx.x & 63, g = l >> 4, j = l & 15;
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) T[4 * g + v][16 * nb + j] = c[nb][v];
-  __syncthreads();
-#pragma unroll
-  for (int f = 0; f < 8; ++f) out[f] = T[j][8 * g + f];
-  __syncthreads();
 }
 
 __device__ __forceinline__ bf16x8 to_bf16x8(const float (&x)[8]) {
@@ -114,7 +128,6 @@ __global__ __launch_bounds__(kFB) void k_gnn_encode(int64_t n, const double *__r
                                                   const float *__restrict__ prm,
                                                   const __bf16 *__restrict__ WqT0,
                                                   __bf16 *__restrict__ h0, __bf16 *__restrict__ Q0) {
-  __shared__ float T[kFB / 64][16][33];
   const Lane ln = lane_of(n);
   const double s = rnorm[0] / sqrt((double)n);
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
@@ -126,16 +139,14 @@ __global__ __launch_bounds__(kFB) void k_gnn_encode(int64_t n, const double *__r
   float hf[8];
 #pragma unroll
   for (int f = 0; f < 8; ++f) {
-    const int F = 8 * ln.g + f;
+    const int F = slot_feature(8 * ln.g + f);
     const float x = ((u0 * prm[F] + u1 * prm[H + F]) + u2 * prm[2 * H + F]) + prm[3 * H + F];
     hf[f] = ln.ok ? q16(relu(x)) : 0.0f;
   }
   bf16x8 a[1] = {to_bf16x8(hf)};
   if (ln.ok) *reinterpret_cast<bf16x8 *>(h0 + ln.v * H + 8 * ln.g) = a[0];
-  floatx4 c[2];
-  mfma_rows<1>(a, WqT0, H, c);
   float qv[8];
-  c_to_rows(c, T[threadIdx.x >> 6], qv);
+  mfma_slots<1>(a, WqT0, H, qv);
   if (ln.ok) *reinterpret_cast<bf16x8 *>(Q0 + ln.v * H + 8 * ln.g) = to_bf16x8(qv);
 }
 
@@ -148,94 +159,115 @@ __global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     __bf16 *__restrict__ h_out, __bf16 *__restrict__ Q_out, float *__restrict__ a_save,
     const float *__restrict__ wout, const double *__restrict__ r,
     const double *__restrict__ d, const double *__restrict__ rnorm, double *__restrict__ z) {
-  __shared__ float T[kFB / 64][16][33];
-  float(*Tw)[33] = T[threadIdx.x >> 6];
   const Lane ln = lane_of(n);
-  const int g = ln.g;
+  const int g = ln.g, j16 = threadIdx.x & 15;
+  // the row's extent and h first: the edge loads wait on rowptr only
   bf16x8 hb = bf16x8{};
-  if (ln.ok) hb = *reinterpret_cast<const bf16x8 *>(h_in + ln.v * H + 8 * g);
-  // P = h Wp
-  floatx4 c[2];
+  int32_t e0 = 0, e1 = 0;
+  if (ln.ok) {
+    e0 = rowptr[ln.v];
+    e1 = rowptr[ln.v + 1];
+    hb = *reinterpret_cast<const bf16x8 *>(h_in + ln.v * H + 8 * g);
+  }
+  // The first kSlots edges: positions clamped to the row, so every load is
+  // valid and all are in flight together. Lane (g, vertex) takes component
+  // g of the edge's features (g = 3: the constant 1 that carries bm).
+  constexpr int kSlots = 6;
+  bf16x8 qs[kSlots];
+  float ef[kSlots];
+  if (e1 > e0) {
+    int32_t es[kSlots], js[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) es[k] = e0 + k < e1 ? e0 + k : e1 - 1;
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) js[k] = col[es[k]];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+      qs[k] = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)js[k] * H + 8 * g);
+      ef[k] = g < 3 ? efeat[3 * (int64_t)es[k] + g] : 1.0f;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+      qs[k] = bf16x8{};
+      ef[k] = 0.0f;
+    }
+  }
+  // P = h Wp (overlaps the edge loads)
+  float P[8];
   {
     bf16x8 a[1] = {hb};
-    mfma_rows<1>(a, lp + kPWp, H, c);
+    mfma_slots<1>(a, lp + kPWp, H, P);
   }
-  float P[8];
-  c_to_rows(c, Tw, P);
-  // messages
-  float we0[8], we1[8], we2[8], bm[8], acc[8];
+  // Messages. The A operand of the edge product: lane (g, m) holds row g of
+  // [We; bm] at output feature 16 nb + m. Every matrix-core instruction runs
+  // on the whole wavefront; only the fold is per row, in edge order.
+  float wc[2];
 #pragma unroll
-  for (int f = 0; f < 8; ++f) {
-    const int F = 8 * g + f;
-    we0[f] = lw[Off::We + F];
-    we1[f] = lw[Off::We + H + F];
-    we2[f] = lw[Off::We + 2 * H + F];
-    bm[f] = lw[Off::bm + F];
-    acc[f] = 0.0f;
+  for (int nb = 0; nb < 2; ++nb)
+    wc[nb] = g < 3 ? lw[Off::We + g * H + 16 * nb + j16] : lw[Off::bm + 16 * nb + j16];
+  const floatx4 Pc[2] = {floatx4{P[0], P[1], P[2], P[3]}, floatx4{P[4], P[5], P[6], P[7]}};
+  float acc[8];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) acc[f] = 0.0f;
+  auto fold = [&](const floatx4 &o0, const floatx4 &o1, const bf16x8 &qj) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      acc[v] = acc[v] + relu(o0[v] + (float)qj[v]);
+      acc[4 + v] = acc[4 + v] + relu(o1[v] + (float)qj[4 + v]);
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const floatx4 o0 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[0], ef[k], Pc[0], 0, 0, 0);
+    const floatx4 o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[1], ef[k], Pc[1], 0, 0, 0);
+    if (e0 + k < e1) fold(o0, o1, qs[k]);
   }
-  if (ln.ok) {
-    const int32_t e0 = rowptr[ln.v], e1 = rowptr[ln.v + 1];
-    auto message = [&](const bf16x8 &qj, float f0, float f1, float f2) {
+  // rows longer than kSlots: a trip count uniform over the wavefront (the
+  // matrix-core instructions need every lane), loads and fold per row
+  int rest = e1 - e0 - kSlots;
 #pragma unroll
-      for (int f = 0; f < 8; ++f) {
-        const float ew = (f0 * we0[f] + f1 * we1[f]) + f2 * we2[f];
-        const float pre = ((P[f] + (float)qj[f]) + ew) + bm[f];
-        acc[f] = acc[f] + relu(pre);
-      }
-    };
-    if (e1 > e0) {
-      // the first kSlots edges: positions clamped to the row, so every load
-      // is valid and all are in flight together; folded in edge order
-      constexpr int kSlots = 6;
-      int32_t es[kSlots], js[kSlots];
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) es[k] = e0 + k < e1 ? e0 + k : e1 - 1;
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) js[k] = col[es[k]];
-      bf16x8 qs[kSlots];
-      float fs[kSlots][3];
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) {
-        qs[k] = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)js[k] * H + 8 * g);
-        fs[k][0] = efeat[3 * (int64_t)es[k]];
-        fs[k][1] = efeat[3 * (int64_t)es[k] + 1];
-        fs[k][2] = efeat[3 * (int64_t)es[k] + 2];
-      }
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k)
-        if (e0 + k < e1) message(qs[k], fs[k][0], fs[k][1], fs[k][2]);
-      for (int32_t e = e0 + kSlots; e < e1; ++e) {
-        const int64_t j = col[e];
-        message(*reinterpret_cast<const bf16x8 *>(Q_in + j * H + 8 * g), efeat[3 * (int64_t)e],
-                efeat[3 * (int64_t)e + 1], efeat[3 * (int64_t)e + 2]);
-      }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int o = __shfl_xor(rest, off, 64);
+    rest = o > rest ? o : rest;
+  }
+  for (int t = 0; t < rest; ++t) {
+    const int32_t e = e0 + kSlots + t;
+    const bool act = e < e1;
+    float eft = 0.0f;
+    bf16x8 qt = bf16x8{};
+    if (act) {
+      eft = g < 3 ? efeat[3 * (int64_t)e + g] : 1.0f;
+      qt = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)col[e] * H + 8 * g);
     }
-    if (a_save) {
+    const floatx4 o0 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[0], eft, Pc[0], 0, 0, 0);
+    const floatx4 o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[1], eft, Pc[1], 0, 0, 0);
+    if (act) fold(o0, o1, qt);
+  }
+  if (a_save && ln.ok) {
 #pragma unroll
-      for (int f = 0; f < 8; ++f) a_save[ln.v * H + 8 * g + f] = acc[f];
-    }
+    for (int f = 0; f < 8; ++f) a_save[ln.v * H + 8 * g + f] = acc[f];
   }
   // t = [h, q(a)] Wu + bu ; h' = q(h + relu(t))
+  float t[8], hn[8];
   {
     bf16x8 a[2] = {hb, to_bf16x8(acc)};
-    mfma_rows<2>(a, lp + kPWu, 2 * H, c);
+    mfma_slots<2>(a, lp + kPWu, 2 * H, t);
   }
-  float t[8], hn[8];
-  c_to_rows(c, Tw, t);
 #pragma unroll
-  for (int f = 0; f < 8; ++f) hn[f] = q16((float)hb[f] + relu(t[f] + lw[Off::bu + 8 * g + f]));
+  for (int f = 0; f < 8; ++f)
+    hn[f] = q16((float)hb[f] + relu(t[f] + lw[Off::bu + slot_feature(8 * g + f)]));
   const bf16x8 hnb = to_bf16x8(hn);
   if (h_out && ln.ok) *reinterpret_cast<bf16x8 *>(h_out + ln.v * H + 8 * g) = hnb;
   if constexpr (!LAST) {
     bf16x8 a[1] = {hnb};
-    mfma_rows<1>(a, WqT_next, H, c);
     float qv[8];
-    c_to_rows(c, Tw, qv);
+    mfma_slots<1>(a, WqT_next, H, qv);
     if (ln.ok) *reinterpret_cast<bf16x8 *>(Q_out + ln.v * H + 8 * g) = to_bf16x8(qv);
   } else {
     float y = 0.0f;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) y = y + hn[f] * wout[8 * g + f];
+    for (int f = 0; f < 8; ++f) y = y + hn[f] * wout[slot_feature(8 * g + f)];
     // the four lanes of a vertex: l, l+16, l+32, l+48
     const int l = threadIdx.x & 63;
     const float y1 = __shfl(y, (l & 15) + 16, 64);
@@ -257,14 +289,15 @@ __global__ void k_gnn_pack(int L, const float *__restrict__ prm, __bf16 *__restr
   const int l = idx / per, k = idx % per;
   const float *lw = prm + o.layer(l);
   float v;
-  if (k < kPWq) {            // WpT[n][k'] = Wp[k'][n]
-    const int nn = k / H, kk = k % H;
+  if (k < kPWq) {            // WpT[n][p] = Wp[F(p)][n]
+    const int nn = k / H, kk = slot_feature(k % H);
     v = lw[Off::Wp + kk * H + nn];
   } else if (k < kPWu) {
-    const int kq = k - kPWq, nn = kq / H, kk = kq % H;
+    const int kq = k - kPWq, nn = kq / H, kk = slot_feature(kq % H);
     v = lw[Off::Wq + kk * H + nn];
-  } else {                   // WuT[n][k'] = Wu[k'][n], k' < 2H
-    const int ku = k - kPWu, nn = ku / (2 * H), kk = ku % (2 * H);
+  } else {                   // WuT[n][32 b + p] = Wu[32 b + F(p)][n], b < 2
+    const int ku = k - kPWu, nn = ku / (2 * H), pk = ku % (2 * H);
+    const int kk = (pk / H) * H + slot_feature(pk % H);
     v = lw[Off::Wu + kk * H + nn];
   }
   packed[idx] = (__bf16)v;
@@ -304,13 +337,13 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_upd(int64_t n, const __bf16 *__r
   if (v >= n) return;
   float x[2 * H], dtv[H];
   for (int k = 0; k < H; ++k) {
-    x[k] = (float)h[v * H + k];
-    x[H + k] = q16(a[v * H + k]);
+    x[k] = (float)h[v * H + feature_slot(k)];
+    x[H + k] = q16(a[v * H + feature_slot(k)]);
   }
   const __bf16 *WuT = lp + kPWu;
   for (int f = 0; f < H; ++f) {
     float t = 0.0f;
-    for (int k = 0; k < 2 * H; ++k) t = t + x[k] * (float)WuT[f * 2 * H + k];
+    for (int k = 0; k < 2 * H; ++k) t = t + x[k] * (float)WuT[f * 2 * H + wu_col(k)];
     t = t + lw[Off::bu + f];
     dtv[f] = t > 0.0f ? gh[v * H + f] : 0.0f;
     dt[v * H + f] = dtv[f];
@@ -318,7 +351,7 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_upd(int64_t n, const __bf16 *__r
   for (int k = 0; k < 2 * H; ++k) {
     xcat[v * 2 * H + k] = x[k];
     float s = 0.0f;
-    for (int f = 0; f < H; ++f) s = s + dtv[f] * (float)WuT[f * 2 * H + k];
+    for (int f = 0; f < H; ++f) s = s + dtv[f] * (float)WuT[f * 2 * H + wu_col(k)];
     if (k < H) gtot[v * H + k] = gh[v * H + k] + s;
     else da[v * H + (k - H)] = s;
   }
@@ -338,11 +371,11 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_edge(int64_t n, const int32_t *_
   const int64_t v = tid_global();
   if (v >= n) return;
   float hv[H], P[H], dPv[H];
-  for (int k = 0; k < H; ++k) hv[k] = (float)h[v * H + k];
+  for (int k = 0; k < H; ++k) hv[k] = (float)h[v * H + feature_slot(k)];
   const __bf16 *WpT = lp + kPWp;
   for (int f = 0; f < H; ++f) {
     float s = 0.0f;
-    for (int k = 0; k < H; ++k) s = s + hv[k] * (float)WpT[f * H + k];
+    for (int k = 0; k < H; ++k) s = s + hv[k] * (float)WpT[f * H + feature_slot(k)];
     P[f] = s;
     dPv[f] = 0.0f;
   }
@@ -351,8 +384,10 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_edge(int64_t n, const int32_t *_
     const float f0 = efeat[3 * (int64_t)e], f1 = efeat[3 * (int64_t)e + 1],
                 f2 = efeat[3 * (int64_t)e + 2];
     for (int f = 0; f < H; ++f) {
-      const float ew = (f0 * lw[Off::We + f] + f1 * lw[Off::We + H + f]) + f2 * lw[Off::We + 2 * H + f];
-      const float pre = ((P[f] + (float)Q[j * H + f]) + ew) + lw[Off::bm + f];
+      // the forward's order: P + e We + bm (matrix core), then + Q_j
+      const float pe = (((P[f] + f0 * lw[Off::We + f]) + f1 * lw[Off::We + H + f]) +
+                        f2 * lw[Off::We + 2 * H + f]) + lw[Off::bm + f];
+      const float pre = pe + (float)Q[j * H + feature_slot(f)];
       const float g = pre > 0.0f ? da[v * H + f] : 0.0f;
       dpre[(int64_t)e * H + f] = g;
       dPv[f] = dPv[f] + g;
@@ -384,8 +419,8 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_node(int64_t n, const int32_t *_
     float s = gtot[v * H + k];
     float sp = 0.0f, sq = 0.0f;
     for (int f = 0; f < H; ++f) {
-      sp = sp + dP[v * H + f] * (float)WpT[f * H + k];
-      sq = sq + q[f] * (float)WqT[f * H + k];
+      sp = sp + dP[v * H + f] * (float)WpT[f * H + feature_slot(k)];
+      sq = sq + q[f] * (float)WqT[f * H + feature_slot(k)];
     }
     gh[v * H + k] = (s + sp) + sq;
   }
@@ -413,6 +448,7 @@ __global__ __launch_bounds__(kB) void k_gnn_bwd_enc(int64_t n, const double *__r
 
 // partial[b][k*N + j] = sum over rows of block b of X[r][k] * D[r][j]
 // (X == nullptr: X = 1, K = 1); rows split into nb equal chunks in order.
+// bfloat16 X is a stored activation (K = H, columns in slot order).
 template <typename TX>
 __global__ __launch_bounds__(kB) void k_outer(int64_t m, const TX *__restrict__ X, int K,
                                              const float *__restrict__ D, int N,
@@ -422,9 +458,10 @@ __global__ __launch_bounds__(kB) void k_outer(int64_t m, const TX *__restrict__ 
   const int64_t lo = b * chunk, hi = lo + chunk < m ? lo + chunk : m;
   for (int idx = threadIdx.x; idx < K * N; idx += kB) {
     const int k = idx / N, j = idx % N;
+    const int kx = std::is_same<TX, __bf16>::value ? feature_slot(k) : k;
     float s = 0.0f;
     for (int64_t r = lo; r < hi; ++r) {
-      const float xv = X ? (float)X[r * K + k] : 1.0f;
+      const float xv = X ? (float)X[r * K + kx] : 1.0f;
       s = s + xv * D[r * N + j];
     }
     partial[b * (int64_t)K * N + idx] = s;

@@ -15,8 +15,9 @@ from oracle import gnn
 pytestmark = pytest.mark.gpu
 
 
-def _setup(n=12, seed=0, L=3):
-    mesh = wmesh.tri2(n, jitter=0.2, seed=seed)
+def _setup(n=12, seed=0, L=3, kind='tri2'):
+    # tri2: rows of at most 6 couplings; tri4: square corners have 8
+    mesh = wmesh.tri2(n, jitter=0.2, seed=seed) if kind == 'tri2' else wmesh.tri4(n)
     rng = np.random.default_rng(seed)
     logk = rng.standard_normal(mesh['points'].shape[0])
     s = wp.p1_system(mesh, wp.kappa_cells(mesh, logk), None)
@@ -47,10 +48,10 @@ def test_untrained_network_is_exactly_jacobi(gpu):
     assert torch.equal(z, r / torch.from_numpy(s['Av']))
 
 
-@pytest.mark.parametrize('seed', [0, 1])
-def test_forward_matches_reference(gpu, seed):
-    print(f'gnn forward seed={seed}', flush=True)
-    s, static, graph, p = _setup(seed=seed)
+@pytest.mark.parametrize('seed,kind', [(0, 'tri2'), (1, 'tri2'), (2, 'tri4')])
+def test_forward_matches_reference(gpu, seed, kind):
+    print(f'gnn forward seed={seed} mesh={kind}', flush=True)
+    s, static, graph, p = _setup(seed=seed, kind=kind)
     pre = LearnedPreconditioner(3, p).bind(graph)
     r = torch.randn(graph.n, dtype=torch.float64,
                     generator=torch.Generator().manual_seed(seed + 3))
@@ -64,8 +65,10 @@ def test_forward_matches_reference(gpu, seed):
     assert _rel(z, zr) < 2e-3
 
 
-def test_backward_matches_reference_autograd(gpu):
-    s, static, graph, p = _setup(seed=2)
+@pytest.mark.parametrize('kind', ['tri2', 'tri4'])
+def test_backward_matches_reference_autograd(gpu, kind):
+    print(f'gnn backward mesh={kind}', flush=True)
+    s, static, graph, p = _setup(seed=2, kind=kind)
     L = 3
     pre = LearnedPreconditioner(L, p).bind(graph, train=True)
     gen = torch.Generator().manual_seed(5)
