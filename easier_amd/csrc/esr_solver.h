@@ -7,9 +7,12 @@ namespace esr {
 // Jacobi/identity-preconditioned CG (easier/numeric/solver/cg.py:74-137).
 // Scalars live on the device so an iteration needs no host round trip; the
 // host polls `done` once per captured chunk of iterations.
+// Fields written by one workgroup of a kernel are never read by the other
+// workgroups of that kernel: a kernel decides into the *_next fields and the
+// next kernel commits them into the *_cur fields.
 struct PcgState {
   double rz_cur;   // sum(r*z) of the current r, z (Step's rzsum)
-  double rz_next;  // sum(r*z) after the next Update (committed by the next step)
+  double rz_next;  // sum(r*z) of the step being run (committed by its second kernel)
   double pAp;
   double alpha;
   double rr;
@@ -25,6 +28,9 @@ struct PcgState {
   int64_t maxiter; // < 0: unset
   int64_t done;    // 1 once the loop has exited
   int64_t iters;   // value of `iters` at exit
+  int64_t fresh;   // 1: the next step starts from p as it is (after Init or a re-arm)
+  int64_t pbuf;    // which of the two direction buffers holds the current p
+  int64_t pbuf_next;
   int64_t pad0;
 };
 

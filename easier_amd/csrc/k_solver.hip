@@ -43,7 +43,8 @@ __global__ __launch_bounds__(kRB) void k_pcg_init(
 }
 
 // bnorm = ||b||; tol = max(rtol*bnorm, atol) if atol else rtol*bnorm
-// (cg.py:112-115); loop counters reset. One workgroup.
+// (cg.py:112-115); loop counters reset; p (in buffer 0) is the first
+// direction as it is. One workgroup.
 __global__ __launch_bounds__(kRB) void k_pcg_init_fin(int64_t g, const double *__restrict__ part,
                                                      PcgState *st) {
   double tot[2];
@@ -59,47 +60,127 @@ __global__ __launch_bounds__(kRB) void k_pcg_init_fin(int64_t g, const double *_
     st->k_next = 0;
     st->done = 0;
     st->iters = 0;
+    st->fresh = 1;
+    st->pbuf = 0;
+    st->pbuf_next = 0;
   }
 }
 
-// Step part 1 (cg.py:44-47): Ap = A p; partials of p.Ap. Workgroup 0 also
-// commits the counters the previous Update produced (nobody else reads them
-// in this kernel).
+// Loop test (cg.py:126-129) of the step just completed, from the partials
+// of r.r and r.z its second kernel stored: rnorm = ||r|| <= tol, or the
+// step cap, or a NaN residual without a cap. On exit the leading thread
+// records the exit (the same values whichever kernel finds it first).
+struct PcgTest {
+  double rr, rz, rnorm;
+  int64_t k;
+  bool exit;
+};
+__device__ __forceinline__ PcgTest pcg_test(const double *__restrict__ part, int64_t g,
+                                            int64_t k, PcgState *st, bool lead) {
+  double s[2];
+  sum_partials<2>(part, g, 0, g, s);
+  PcgTest t;
+  t.rr = s[0];
+  t.rz = s[1];
+  t.rnorm = cr_sqrt(s[0]);
+  t.k = k;
+  const bool converged = (t.rnorm == t.rnorm) && t.rnorm <= st->tol;
+  const bool capped = st->maxiter >= 0 && t.k >= st->maxiter;
+  const bool nan_stop = t.rnorm != t.rnorm && st->maxiter < 0;
+  t.exit = converged || capped || nan_stop;
+  if (t.exit && lead) {
+    st->rr = t.rr;
+    st->rz_new = t.rz;
+    st->rnorm = t.rnorm;
+    st->done = 1;
+    st->iters = t.k;
+  }
+  return t;
+}
+
+// One PCG iteration is two kernels on small systems, three on large ones
+// (kFuseUpdateMax); both forms run the same operations on the same values.
+//
+// Kernel 1: the loop test and Update (cg.py:67-71, 126-129) of the previous
+// step -- beta = rz_new / rz and p = z + beta p -- fused with Step part 1
+// (cg.py:44-47), Ap = A p and partials of p.Ap. The new direction is
+// evaluated where it is used, z[c] + beta p[c] for the row and every
+// neighbour (the same operations, so the same bits, as updating p first),
+// read from one direction buffer and written to the other, so no workgroup
+// overwrites a value another one still gathers. After Init, a re-arm or the
+// separate Update kernel (fresh) the step starts from p as it is.
 template <typename T>
-__global__ __launch_bounds__(kRB) void k_pcg_spmv(
+__global__ __launch_bounds__(kRB) void k_pcg_step1(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ p,
-    T *__restrict__ Ap, PcgState *st, double *__restrict__ part) {
+    const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ z,
+    T *__restrict__ p0, T *__restrict__ p1, T *__restrict__ Ap, PcgState *st,
+    const double *__restrict__ part_in, double *__restrict__ part_out) {
   if (st->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->k_cur = st->k_next;
-    st->rz_cur = st->rz_next;
-  }
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int64_t cur = st->pbuf;
+  const T *__restrict__ pin = cur ? p1 : p0;
+  T *__restrict__ pout = cur ? p0 : p1;
   double acc[1] = {0.0};
-  ESR_GRID_ROWS(i, n) {
-    const T y = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
-    Ap[i] = y;
-    acc[0] += (double)(p[i] * y);
+  if (st->fresh) {
+    if (lead) {
+      st->k_next = st->k_cur;
+      st->rz_next = st->rz_cur;
+      st->pbuf_next = cur;
+    }
+    ESR_GRID_ROWS(i, n) {
+      const T y = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return pin[c]; });
+      Ap[i] = y;
+      acc[0] += (double)(pin[i] * y);
+    }
+  } else {
+    const PcgTest t = pcg_test(part_in, gridDim.x, st->k_cur, st, lead);
+    if (t.exit) return;
+    const T beta = (T)(t.rz / st->rz_cur);
+    if (lead) {
+      st->rr = t.rr;
+      st->rz_new = t.rz;
+      st->rnorm = t.rnorm;
+      st->k_next = t.k + 1;
+      st->rz_next = t.rz;
+      st->beta = (double)beta;
+      st->pbuf_next = 1 - cur;
+    }
+    ESR_GRID_ROWS(i, n) {
+      const T y = linsys_row<T>(i, rowptr, col, Ae, Av,
+                                [&](int64_t c) { return z[c] + beta * pin[c]; });
+      const T pi = z[i] + beta * pin[i];
+      pout[i] = pi;
+      Ap[i] = y;
+      acc[0] += (double)(pi * y);
+    }
   }
-  store_partials<1>(acc, part);
+  store_partials<1>(acc, part_out);
 }
 
-// Step part 2 (cg.py:47-52) fused with the first half of Update (cg.py:67):
-// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M r; partials of r.r, r.z.
+// Kernel 2: Step part 2 (cg.py:47-52) fused with the first half of Update
+// (cg.py:67): alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M r;
+// partials of r.r, r.z. Workgroup 0 commits what kernel 1 decided.
 template <typename T>
-__global__ __launch_bounds__(kRB) void k_pcg_xr(int64_t n, const T *__restrict__ diag,
-                                               T *__restrict__ x, T *__restrict__ r,
-                                               T *__restrict__ z, const T *__restrict__ p,
-                                               const T *__restrict__ Ap, PcgState *st,
-                                               const double *__restrict__ part_in,
-                                               double *__restrict__ part_out) {
+__global__ __launch_bounds__(kRB) void k_pcg_step2(int64_t n, const T *__restrict__ diag,
+                                                  T *__restrict__ x, T *__restrict__ r,
+                                                  T *__restrict__ z, const T *__restrict__ p0,
+                                                  const T *__restrict__ p1,
+                                                  const T *__restrict__ Ap, PcgState *st,
+                                                  const double *__restrict__ part_in,
+                                                  double *__restrict__ part_out) {
   if (st->done) return;
   double pAp[1];
   sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, pAp);
-  const double alpha_d = st->rz_cur / pAp[0];
+  const double rz = st->rz_next;
+  const double alpha_d = rz / pAp[0];
+  const T *__restrict__ p = st->pbuf_next ? p1 : p0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pAp = pAp[0];
     st->alpha = alpha_d;
+    st->k_cur = st->k_next;
+    st->rz_cur = rz;
+    st->pbuf = st->pbuf_next;
+    st->fresh = 0;
   }
   const T alpha = (T)alpha_d;
   double acc[2] = {0.0, 0.0};
@@ -116,49 +197,56 @@ __global__ __launch_bounds__(kRB) void k_pcg_xr(int64_t n, const T *__restrict__
   store_partials<2>(acc, part_out);
 }
 
-// Loop test (cg.py:126-129) and Update (cg.py:67-71): rnorm = ||r||;
-// stop, or beta = rz_new / rz and p = z + beta p. Once the loop has exited
-// the two kernels before this one do nothing, and this one re-derives the
-// same exit from the unchanged partials (idempotent).
+// Kernel 3 of the three-kernel form: the loop test and Update of the step
+// just completed, p = z + beta p in place. It reads the *_next copies (equal
+// to *_cur after kernel 2) and writes *_cur, so the next kernel 1 starts
+// fresh from the updated p.
 template <typename T>
-__global__ __launch_bounds__(kRB) void k_pcg_p(int64_t n, const T *__restrict__ z,
-                                              T *__restrict__ p, PcgState *st,
-                                              const double *__restrict__ part_in) {
-  double s[2];
-  sum_partials<2>(part_in, gridDim.x, 0, gridDim.x, s);
-  const double rnorm = cr_sqrt(s[0]);
-  const int64_t k = st->k_cur;
-  const bool converged = (rnorm == rnorm) && rnorm <= st->tol;
-  const bool capped = st->maxiter >= 0 && k >= st->maxiter;
-  const bool nan_stop = rnorm != rnorm && st->maxiter < 0;
+__global__ __launch_bounds__(kRB) void k_pcg_update(int64_t n, const T *__restrict__ z,
+                                                   T *__restrict__ p0, T *__restrict__ p1,
+                                                   PcgState *st,
+                                                   const double *__restrict__ part_in) {
+  if (st->done) return;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (converged || capped || nan_stop) {
-    if (lead) {
-      st->rr = s[0];
-      st->rz_new = s[1];
-      st->rnorm = rnorm;
-      st->done = 1;
-      st->iters = k;
-    }
-    return;
-  }
-  const T beta = (T)(s[1] / st->rz_cur);
+  const PcgTest t = pcg_test(part_in, gridDim.x, st->k_next, st, lead);
+  if (t.exit) return;
+  const T beta = (T)(t.rz / st->rz_next);
+  T *__restrict__ p = st->pbuf_next ? p1 : p0;
   ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
   if (lead) {
-    st->rr = s[0];
-    st->rz_new = s[1];
-    st->rnorm = rnorm;
-    st->k_next = k + 1;
-    st->rz_next = s[1];
+    st->rr = t.rr;
+    st->rz_new = t.rz;
+    st->rnorm = t.rnorm;
     st->beta = (double)beta;
+    st->k_cur = t.k + 1;
+    st->rz_cur = t.rz;
+    st->fresh = 1;
   }
+}
+
+// End of a batch in the two-kernel form: the loop test of the last step, so
+// that the host sees `done` without another batch (kernel 1 of a following
+// step re-derives the same exit from the same partials), and once the loop
+// has exited the last direction is copied into the caller's p if it lives
+// in the second buffer (the reference's p after the loop; no workgroup
+// changes pbuf, so all decide alike).
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_pcg_settle(int64_t n, T *__restrict__ p0,
+                                                   const T *__restrict__ p1,
+                                                   PcgState *st,
+                                                   const double *__restrict__ part) {
+  if (st->fresh) return;
+  const bool exited =
+      st->done || pcg_test(part, gridDim.x, st->k_cur, st, blockIdx.x == 0 && threadIdx.x == 0).exit;
+  if (exited && st->pbuf == 1) ESR_GRID_ROWS(i, n) p0[i] = p1[i];
 }
 
 // Re-arm after a reliable update (the caller replaced r by the true residual
 // of the current iterate): z = M r and partials of r.z. k_pcg_rearm_fin
 // takes the Update of cg.py:67-71 with this residual -- beta = r.z / rz of
 // the last step -- and installs the new exit tolerance; k_pcg_rearm_p sets
-// p = z + beta p, so the search directions stay conjugate.
+// p = z + beta p in the buffer that holds p, so the search directions stay
+// conjugate, and the next step starts from it as it is.
 template <typename T>
 __global__ __launch_bounds__(kRB) void k_pcg_rearm(int64_t n, const T *__restrict__ diag,
                                                   const T *__restrict__ r, T *__restrict__ z,
@@ -180,17 +268,23 @@ __global__ __launch_bounds__(kRB) void k_pcg_rearm_fin(int64_t g, const double *
   if (threadIdx.x == 0) {
     st->beta = tot[0] / st->rz_cur;
     st->rz_new = tot[0];
+    st->rz_cur = tot[0];
     st->rz_next = tot[0];
     st->tol = tol;
-    st->k_next = st->k_cur + 1;
+    st->k_cur = st->k_cur + 1;
+    st->k_next = st->k_cur;
+    st->fresh = 1;
+    st->pbuf_next = st->pbuf;
     st->done = 0;
   }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kRB) void k_pcg_rearm_p(int64_t n, const T *__restrict__ z,
-                                                    T *__restrict__ p, const PcgState *st) {
+                                                    T *__restrict__ p0, T *__restrict__ p1,
+                                                    const PcgState *st) {
   const T beta = (T)st->beta;
+  T *__restrict__ p = st->pbuf ? p1 : p0;
   ESR_GRID_ROWS(i, n) p[i] = z[i] + beta * p[i];
 }
 
@@ -424,13 +518,26 @@ using namespace esr;
     }                                                                           \
   } while (0)
 
-// Workspace of the PCG kernels: partials of Init (2 G), Step part 1 (G) and
-// Step part 2 (2 G), G = red_grid(n) workgroups; part 1 and part 2 live in
-// different regions so a kernel never overwrites what it is still summing.
+// Workspace of the PCG kernels: partials of Init (2 G), Step kernel 1 (G)
+// and kernel 2 (2 G), G = red_grid(n) workgroups -- kernel 1's and kernel
+// 2's in different regions, so a kernel never overwrites what it is still
+// summing -- then, 256-byte aligned, the second direction buffer (n values
+// of up to 8 bytes).
+// Largest system whose PCG iteration is two kernels: below it the
+// iteration is launch-bound and one kernel fewer pays; above it the doubled
+// neighbour gathers of the fused Update cost more than a separate pass over
+// z and p (measured at 10,201 vertices: 11.8 -> 10.8 us per iteration; at
+// 4.2 M: 170 -> 180 us).
+constexpr int64_t kFuseUpdateMax = 1 << 17;
+
+static int64_t pcg_parts_bytes(int64_t n) {
+  return (3 * (int64_t)red_grid(n) * (int64_t)sizeof(double) + 255) / 256 * 256;
+}
+
 ESR_API int esr_pcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && state_bytes && ws_bytes, "esr_pcg_workspace: bad args");
   *state_bytes = (int64_t)sizeof(PcgState);
-  *ws_bytes = 3 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
+  *ws_bytes = pcg_parts_bytes(n) + 8 * n;
   return ESR_OK;
 }
 
@@ -471,15 +578,25 @@ ESR_API int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr, const int
   hipStream_t s = as_stream(stream);
   const unsigned g = red_grid(n);
   double *part1 = (double *)ws, *part2 = part1 + g;
+  void *p1 = (char *)ws + pcg_parts_bytes(n);
+  const bool fuse = n <= kFuseUpdateMax;
   for (int64_t it = 0; it < count; ++it) {
     ESR_FLOAT_DISPATCH(dtype, {
-      k_pcg_spmv<T><<<g, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
-                                      (const T *)p, (T *)Ap, (PcgState *)state, part1);
-      k_pcg_xr<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
-                                    (const T *)p, (const T *)Ap, (PcgState *)state, part1,
-                                    part2);
-      k_pcg_p<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (PcgState *)state, part2);
+      k_pcg_step1<T><<<g, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
+                                       (const T *)z, (T *)p, (T *)p1, (T *)Ap,
+                                       (PcgState *)state, part2, part1);
+      k_pcg_step2<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
+                                       (const T *)p, (const T *)p1, (const T *)Ap,
+                                       (PcgState *)state, part1, part2);
+      if (!fuse)
+        k_pcg_update<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (T *)p1, (PcgState *)state,
+                                          part2);
     });
+    ESR_LAUNCHED();
+  }
+  if (count > 0 && fuse) {
+    ESR_FLOAT_DISPATCH(dtype, (k_pcg_settle<T><<<g, kRB, 0, s>>>(n, (T *)p, (const T *)p1,
+                                                                 (PcgState *)state, part2)));
     ESR_LAUNCHED();
   }
   return ESR_OK;
@@ -498,7 +615,9 @@ ESR_API int esr_pcg_rearm(int dtype, int64_t n, const void *diag, const void *r,
   ESR_FLOAT_DISPATCH(dtype, {
     k_pcg_rearm<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (const T *)r, (T *)z, (double *)ws);
     k_pcg_rearm_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (PcgState *)state, tol);
-    k_pcg_rearm_p<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (const PcgState *)state);
+    k_pcg_rearm_p<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p,
+                                       (T *)((char *)ws + pcg_parts_bytes(n)),
+                                       (const PcgState *)state);
   });
   ESR_LAUNCHED();
   return ESR_OK;

@@ -241,7 +241,11 @@ int esr_p1_assemble(int64_t np_, int64_t nc, const int64_t *cells,
  * (esr_pcg_workspace sizes it): esr_pcg_init runs Init and fixes
  * tol = max(rtol*||b||, atol) if atol > 0 else rtol*||b|| and maxiter (< 0:
  * none); esr_pcg_steps enqueues `count` iterations of Step + loop test +
- * Update, each a no-op once the loop has exited (safe to capture in a graph);
+ * Update (two kernels each, the Update of one step fused with the next
+ * step's product on systems of at most 2^17 rows; the workspace holds the
+ * second direction buffer) and one final loop test that also leaves the last
+ * direction in p once the loop has exited, each a no-op once the loop has
+ * exited (safe to capture in a graph);
  * esr_pcg_read copies scalars {rnorm, bnorm, rz, tol, alpha, beta} and ints
  * {iters, done, k} to the host. dtype: ESR_F64 or ESR_F32 (dots always
  * accumulate in double).                                                     */

@@ -23,9 +23,12 @@ class Linsys:
 
 
 def cg(A: Callable, b: torch.Tensor, x: torch.Tensor,
-       M: Optional[Callable] = None, rtol=1e-5, atol=None, maxiter=None):
+       M: Optional[Callable] = None, rtol=1e-5, atol=None, maxiter=None,
+       final: Optional[dict] = None):
     """numeric/solver/cg.py:25-137. Returns (info, x) with info =
-    {'residual', 'iters'}; x is updated in place."""
+    {'residual', 'iters'}; x is updated in place. `final`, if given, receives
+    the solver's r, p and rzsum when the loop exits (before the Update it
+    skips)."""
     M = M if M else (lambda v: v)
     # Init (cg.py:25-29)
     r = b - A(x)
@@ -54,6 +57,8 @@ def cg(A: Callable, b: torch.Tensor, x: torch.Tensor,
         beta = aggregate('sum', r * z1).sum() / rzsum
         p = z1 + beta * p
         z = z1
+    if final is not None:
+        final.update(r=r, p=p, rzsum=float(rzsum))
     return {'residual': float(rnorm), 'iters': iters}, x
 
 

@@ -77,6 +77,31 @@ def test_cg_matches_oracle(gpu, precond, reorder):
     torch.testing.assert_close(eqn.x.collect(), x_o, rtol=1e-8, atol=1e-10)
 
 
+@pytest.mark.parametrize('m', [24, 370])
+def test_cg_final_state_matches_oracle(gpu, m):
+    """The fused CG leaves the module's r, p and rzsum as the reference loop
+    does at exit: 24^2 unknowns run the two-kernel iteration (p alternates
+    between two buffers and is copied back at exit), 370^2 = 136,900 the
+    three-kernel one."""
+    print(f'cg final state m={m}', flush=True)
+    eqn = Poisson5(m, seed=3)
+    A_o, b_o = eqn.oracle()
+    sol = CG(eqn.A, eqn.b, eqn.x, Jacobi(eqn.Ac))
+    esr.compile([eqn, sol], 'hip')
+    info = sol.solve(rtol=1e-10, maxiter=2000)
+    fin = {}
+    x_o = torch.zeros_like(b_o)
+    info_o, x_o = osol.cg(A_o, b_o, x_o, M=lambda r: r / A_o.Av, rtol=1e-10, maxiter=2000,
+                          final=fin)
+    assert info['iters'] == info_o['iters'], (info, info_o)
+    torch.testing.assert_close(eqn.x.collect(), x_o, rtol=1e-8, atol=1e-10)
+    for name in ('r', 'p'):
+        got, want = getattr(sol, name).collect(), fin[name]
+        err = float((got - want).norm() / want.norm())
+        assert err < 1e-6, (name, err)
+    assert abs(float(sol.rzsum.collect()[0]) - fin['rzsum']) <= 1e-6 * abs(fin['rzsum'])
+
+
 def test_gmres_matches_oracle(gpu):
     eqn = Poisson5(12, seed=1)
     A_o, b_o = eqn.oracle()
