@@ -102,14 +102,19 @@ def test_cg_final_state_matches_oracle(gpu, m):
     assert abs(float(sol.rzsum.collect()[0]) - fin['rzsum']) <= 1e-6 * abs(fin['rzsum'])
 
 
-def test_gmres_matches_oracle(gpu):
-    eqn = Poisson5(12, seed=1)
+@pytest.mark.parametrize('m,precond', [(12, False), (12, True), (140, False), (140, True)])
+def test_gmres_matches_oracle(gpu, m, precond):
+    """144 and 19,600 unknowns, identity and Jacobi preconditioner."""
+    print(f'gmres m={m} precond={precond}', flush=True)
+    eqn = Poisson5(m, seed=1)
     A_o, b_o = eqn.oracle()
-    sol = GMRES(eqn.A, eqn.b, eqn.x, restart=10)
+    M = Jacobi(eqn.Ac) if precond else None
+    sol = GMRES(eqn.A, eqn.b, eqn.x, M, restart=10)
     esr.compile([eqn, sol], 'hip')
     info = sol.solve(rtol=1e-9, maxiter=200)
     x_o = torch.zeros_like(b_o)
-    info_o, x_o = osol.gmres(A_o, b_o, x_o, restart=10, rtol=1e-9, maxiter=200)
+    info_o, x_o = osol.gmres(A_o, b_o, x_o, M=(lambda r: r / A_o.Av) if precond else None,
+                             restart=10, rtol=1e-9, maxiter=200)
     assert abs(info['iters'] - info_o['iters']) <= 1
     torch.testing.assert_close(eqn.x.collect(), x_o, rtol=1e-7, atol=1e-9)
     assert info['residual'] < 1e-9 * float(torch.norm(b_o)) * 1.0001
