@@ -228,6 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_seq(const T *__restrict__ vals, 
 // then every lane folds its own row from LDS in entry order -- the same
 // sequential sum as k_csr_seq, with a third of the address work for rows of
 // ~3 entries. Segments longer than kCap entries fall back to direct loads.
+// (128 rows per wavefront, two per lane, measured slower: 59.8 vs 54.4 us
+// for the 8.4 M-row, 25.2 M-entry float64 sum.)
 template <typename T, int OP, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ vals,
                                                       const int32_t *__restrict__ rowptr,
