@@ -44,6 +44,18 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return start + pos;
 }
 
+// Workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th: groups of G
+// consecutive tiles go to one XCD, XCDs taking consecutive groups, so the 8
+// XCDs stream through one window of 8 G tiles together while a tile's
+// neighbours mostly stay in its XCD's L2. Tiles past the last whole window
+// keep their own index.
+__device__ __forceinline__ int64_t group_remap(int64_t b, int64_t nb, int64_t G) {
+  const int64_t full = (nb / (kNumXcd * G)) * kNumXcd * G;
+  if (b >= full) return b;
+  const int64_t x = b % kNumXcd, s = b / kNumXcd;
+  return ((s / G) * kNumXcd + x) * G + (s % G);
+}
+
 // IEEE correctly rounded square root. The device sqrt can be off by one ulp;
 // given y within one ulp of sqrt(x), the Tuckerman rounding test (Markstein,
 // "IA-64 and Elementary Functions", 2000) picks the correctly rounded

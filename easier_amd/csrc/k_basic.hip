@@ -105,7 +105,11 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const U *__restrict__ src
                                                       U *__restrict__ dst, int64_t n) {
   static_assert(sizeof(U) * V == 16, "16-byte output vectors");
   typedef typename GatherIdx<V>::I IV;
-  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  // groups of 64 consecutive tiles per XCD: the gathered neighbours of a
+  // tile mostly sit in its XCD's L2, and the 8 XCDs still stream through one
+  // window together. Measured on the 25.2 M-edge float64 gather: 57.2 us;
+  // one contiguous range per XCD (xcd_remap) 59.4 us; launch order 63.2 us.
+  const int64_t tile = group_remap(blockIdx.x, gridDim.x, 64);
   const int64_t nvec = n / V;
   int64_t vi[2];
   IV ii[2];
@@ -240,7 +244,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_staged(const T *__restrict__ val
   typedef T tv_t __attribute__((ext_vector_type(V)));
   __shared__ T stage[kBlock / 64][kCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  // Tiles in launch order: the segment sum reuses nothing across tiles, and
+  // the 8 XCDs streaming through one region together beat one contiguous
+  // range per XCD (xcd_remap): 51.9 vs 54.0 us for the 8.4 M-row, 25.2 M-entry
+  // float64 sum (groups of 16 or 64 tiles per XCD: 52.1 us).
+  const int64_t tile = blockIdx.x;
   const int64_t r0 = (tile * (kBlock / 64) + w) * 64;
   if (r0 >= n_rows) return;                       // the whole wavefront
   const int64_t r = r0 + lane;
