@@ -25,6 +25,9 @@ inline unsigned red_grid(int64_t n) {
 }
 
 // Rows of a grid-stride kernel: i = blockIdx.x * kRB + threadIdx.x + k * stride.
+// (A contiguous chunk of rows per workgroup, which keeps a row's neighbours
+// in its XCD's L2, measured slower: Jacobi-PCG at 4.2 M vertices 1.62 vs
+// 1.54 s, the 8 XCDs then stream from 8 distant regions.)
 #define ESR_GRID_ROWS(i, n)                                                    \
   for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < (n);           \
        i += (int64_t)gridDim.x * kRB)
