@@ -149,6 +149,26 @@ def test_aggregators(gpu, op, n):
     assert torch.equal(a, a2)
 
 
+@pytest.mark.parametrize('op', ['sum', 'norm', 'max', 'min'])
+@pytest.mark.parametrize('n', [1, 3, 777, 300_001])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_aggregators_one_column(gpu, op, n, dtype):
+    """One column, float64 and float32, aligned and offset views, lengths
+    that leave every remainder of a 16-byte vector."""
+    g = torch.Generator().manual_seed(8)
+    t = torch.randn(n + 1, generator=g, dtype=torch.float64).to(dtype)
+    tol = 1e-12 if dtype == torch.float64 else 1e-4
+    dev = t.to(gpu)
+    for host, view in ((t[:n], dev[:n]), (t[1:], dev[1:])):
+        a = ops.aggregate(op, view.reshape(-1, 1)).cpu().reshape(-1)
+        b = oops.aggregate(op, host.reshape(-1, 1).double()).reshape(-1)
+        if op in ('max', 'min'):
+            assert float(a[0]) == float(b[0])
+        else:
+            assert abs(float(a[0]) - float(b[0])) <= tol * max(abs(float(b[0])), 1.0)
+        assert torch.equal(a, ops.aggregate(op, view.reshape(-1, 1)).cpu().reshape(-1))
+
+
 @pytest.mark.parametrize('op', ['add', 'sub', 'mul', 'div'])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_ewise_arithmetic_bit_exact(gpu, op, dtype):
