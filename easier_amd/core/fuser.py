@@ -75,7 +75,9 @@ class Unsupported(Exception):
 #                  (bytes fetched from beyond L2 / algorithmic, ms per step):
 #                  no remap 1.285 / 1.189, xcd_remap 1.002 / 1.217,
 #                  G = 16 1.105 / 1.164, G = 64 1.035 / 1.175, G = 256
-#                  1.009 / 1.196.
+#                  1.009 / 1.196; measured again with the current kernels
+#                  (ms per step, two runs each): G = 8 1.163, G = 16
+#                  1.161 / 1.167, G = 32 1.179 / 1.163, G = 64 1.177 / 1.180.
 #   rows_per_thread - 2: a thread evaluates rows r0 and r0 + 1 (r0 even); the
 #                  row arrays are read and written as 2-element vectors (16
 #                  bytes in float64), the edge loops stay per row. Measured on
@@ -97,7 +99,7 @@ class Unsupported(Exception):
 #                  256/318/318 us and made the fourth spill 44 bytes per lane
 #                  (292 -> 417 us, step 1.18 -> 1.31 ms), so the default is 0.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
-           'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 64,
+           'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 16,
            'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0}
 _LDS_BUDGET = 48 * 1024
 
