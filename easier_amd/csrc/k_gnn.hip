@@ -129,19 +129,31 @@ __global__ __launch_bounds__(kFB) void k_gnn_encode(int64_t n, const double *__r
                                                   const __bf16 *__restrict__ WqT0,
                                                   __bf16 *__restrict__ h0, __bf16 *__restrict__ Q0) {
   const Lane ln = lane_of(n);
-  const double s = rnorm[0] / sqrt((double)n);
-  float u0 = 0.f, u1 = 0.f, u2 = 0.f;
+  const int g = ln.g, j16 = threadIdx.x & 15;
+  // h0 = q(relu([u, 1] [W_in; b_in])) as one float32 matrix-core product
+  // per 16 output features: lane (g, vertex) supplies component g of
+  // u = (r / s, vfeat) (g = 3: the constant 1 that carries b_in), lane
+  // (g, m) row g of [W_in; b_in] at feature 16 nb + m; the result is in
+  // slot layout.
+  float ug = 0.0f;
   if (ln.ok) {
-    u0 = s > 0.0 ? (float)(r[ln.v] / s) : 0.0f;
-    u1 = vfeat[2 * ln.v];
-    u2 = vfeat[2 * ln.v + 1];
+    if (g == 0) {
+      const double s = rnorm[0] / sqrt((double)n);
+      ug = s > 0.0 ? (float)(r[ln.v] / s) : 0.0f;
+    } else if (g < 3) {
+      ug = vfeat[2 * ln.v + (g - 1)];
+    } else {
+      ug = 1.0f;
+    }
   }
   float hf[8];
 #pragma unroll
-  for (int f = 0; f < 8; ++f) {
-    const int F = slot_feature(8 * ln.g + f);
-    const float x = ((u0 * prm[F] + u1 * prm[H + F]) + u2 * prm[2 * H + F]) + prm[3 * H + F];
-    hf[f] = ln.ok ? q16(relu(x)) : 0.0f;
+  for (int nb = 0; nb < 2; ++nb) {
+    const float wa = prm[g * H + 16 * nb + j16];   // rows 0..2: W_in, row 3: b_in
+    const floatx4 c = __builtin_amdgcn_mfma_f32_16x16x4f32(wa, ug, floatx4{0.f, 0.f, 0.f, 0.f},
+                                                           0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) hf[4 * nb + v] = ln.ok ? q16(relu(c[v])) : 0.0f;
   }
   bf16x8 a[1] = {to_bf16x8(hf)};
   if (ln.ok) *reinterpret_cast<bf16x8 *>(h0 + ln.v * H + 8 * ln.g) = a[0];
