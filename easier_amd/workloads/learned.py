@@ -22,7 +22,7 @@ import torch
 
 from .. import _native as nat
 from ..core import ops
-from ..numeric.preconditioner.learned import LearnedPreconditioner, MeshGraph
+from ..numeric.preconditioner.learned import GnnBuffers, LearnedPreconditioner, MeshGraph
 from ..numeric.solver import fused as _fused  # noqa: F401  (registers esr_pcg_*)
 from . import poisson as wp
 
@@ -436,8 +436,30 @@ def _log(msg):
     print(f'[learned {time.strftime("%H:%M:%S")}] {msg}', file=sys.stderr, flush=True)
 
 
+def time_forward(pre: LearnedPreconditioner, n, seed=1, reps=50):
+    """Microseconds per network application z = M(r) (||r||, encoder and the
+    layers) on the n x n problem of `seed`, HIP events over `reps`
+    applications after 3 warm-up ones, r uniform in [0, 1)."""
+    prob = Problem(n, seed)
+    pre.bind(prob.graph)
+    bufs = GnnBuffers(prob.n, prob.device)
+    g = torch.Generator(device='cpu').manual_seed(0)
+    r = torch.rand(prob.n, generator=g, dtype=torch.float64).to(prob.device)
+    z = torch.empty_like(r)
+    for _ in range(3):
+        pre.forward_into(prob.graph, bufs, r, z)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        pre.forward_into(prob.graph, bufs, r, z)
+    b.record()
+    torch.cuda.synchronize()
+    return dict(n_grid=n, vertices=prob.n, seed=seed, reps=reps,
+                us_per_application=a.elapsed_time(b) * 1e3 / reps)
+
+
 def main(argv=None):
-    """python -m easier_amd.workloads.learned train|evaluate ..."""
+    """python -m easier_amd.workloads.learned train|evaluate|forward ..."""
     import argparse
     from ..numeric.preconditioner import learned as lp
     ap = argparse.ArgumentParser(prog='python -m easier_amd.workloads.learned')
@@ -473,8 +495,18 @@ def main(argv=None):
                          'the network as smoother')
     ev.add_argument('--learned-smoothing', choices=['pre', 'post', 'both'], default='pre',
                     help='where the network smooths on those levels (damped Jacobi elsewhere)')
+    fw = sub.add_parser('forward', help='time network applications z = M(r) on the '
+                        'n x n problem of a seed (vertices in natural order)')
+    fw.add_argument('--weights', required=True)
+    fw.add_argument('--n', type=int, default=2048)
+    fw.add_argument('--seed', type=int, default=1)
+    fw.add_argument('--reps', type=int, default=50)
     args = ap.parse_args(argv)
     import json
+    if args.cmd == 'forward':
+        print(json.dumps(time_forward(lp.load(args.weights), args.n, args.seed, args.reps)),
+              flush=True)
+        return
     if args.cmd == 'train':
         data, t0 = [], time.perf_counter()
         sizes = [args.n[k % len(args.n)] for k in range(args.problems)]
