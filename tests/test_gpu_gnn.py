@@ -115,3 +115,11 @@ def test_autograd_function_and_finite_differences(gpu):
     fd = (vals[0] - vals[1]) / (2 * eps)
     an = float((g_auto * v).sum())
     assert abs(fd - an) <= 0.1 * abs(an), (fd, an)
+
+
+def test_forward_timing_command(gpu):
+    """`python -m easier_amd.workloads.learned forward` on a small mesh."""
+    from easier_amd.workloads import learned as wl
+    pre = LearnedPreconditioner(3, gnn.init_params(3, 0))
+    res = wl.time_forward(pre, 32, seed=2, reps=3)
+    assert res['vertices'] == 33 * 33 and res['us_per_application'] > 0
