@@ -225,3 +225,31 @@ def test_gemv_rows(gpu):
     y = torch.randn((20, 1), generator=g, dtype=torch.float64)
     out = ops.gemv_rows(V[:, :7].to(gpu), y[:7].to(gpu)).cpu()
     torch.testing.assert_close(out, (V[:, :7] @ y[:7]).squeeze(-1), rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_primitives_at_full_size(gpu, dtype):
+    """The benchmark's primitives at their size, bit for bit against the
+    oracle: the cell -> edge gather of the 2048 x 2048 2-triangle mesh's
+    25,157,632 directed dual edges (16-byte vector path) and the edge -> cell
+    sum over the edges sorted by destination (LDS-staged path), one value per
+    edge, 8,388,608 rows."""
+    from easier_amd.workloads import mesh as wmesh
+    print(f'primitives full size {dtype}', flush=True)
+    mesh = wmesh.tri2(2048)
+    src = torch.as_tensor(mesh['src']).long()
+    dst = torch.as_tensor(mesh['dst']).long()
+    nc = mesh['cells'].shape[0]
+    order = torch.sort(dst, stable=True).indices   # the compiled edge order
+    src, dst = src[order], dst[order]
+    g = torch.Generator().manual_seed(11)
+    cells = torch.randn(nc, generator=g, dtype=torch.float64).to(dtype)
+    got = ops.gather(cells.to(gpu), src.to(torch.int32).to(gpu)).cpu()
+    want = oops.selector(cells, src)
+    assert torch.equal(got, want)
+    counts = torch.bincount(dst, minlength=nc)
+    rowptr = torch.cat([torch.zeros(1, dtype=torch.long), torch.cumsum(counts, 0)]).to(torch.int32)
+    lr = torch.zeros(0, dtype=torch.int32)
+    out = ops.csr_reduce(got.to(gpu), rowptr.to(gpu), None, lr.to(gpu), nc, 'sum').cpu()
+    ref = oops.reducer(want, dst, nc)
+    assert torch.equal(out, ref)
