@@ -3,19 +3,27 @@ with a variable coefficient (4,198,401 vertices, 25,174,016 couplings), f64,
 relative tolerance 1e-8. The oracle's CG would need minutes on the host, so
 the checks are size-independent: the true residual ||b - A x|| recomputed on
 the host with scipy, and two independent methods (fused Jacobi-PCG, the
-three-kernel form at this size, and multigrid flexible CG) agreeing on x."""
+three-kernel form at this size, and multigrid flexible CG with damped-Jacobi
+or the trained network as smoother) agreeing on x."""
+import os
+
 import pytest
 
+from easier_amd.numeric.preconditioner import learned as lp
 from easier_amd.workloads import learned as wl
+
+WEIGHTS = os.path.join(os.path.dirname(wl.__file__), 'gnn_smoother_l3_h32.npz')
 
 pytestmark = pytest.mark.gpu
 
 
 def test_full_size_jacobi_pcg_and_multigrid_fcg_agree(gpu):
     print('full-size Poisson 2048x2048: Jacobi-PCG and multigrid FCG', flush=True)
-    res = wl.evaluate_compiled(None, 2048, 1, rtol=1e-8, maxiter=20000, reps=1,
-                               methods=('jacobi', 'mg'))
-    jac, mg = res['jacobi'], res['mg']
+    res = wl.evaluate_compiled(lp.load(WEIGHTS), 2048, 1, rtol=1e-8, maxiter=20000, reps=1,
+                               methods=('jacobi', 'mg', 'mg-learned'))
+    jac, mg, ml = res['jacobi'], res['mg'], res['mg-learned']
+    assert ml['true_rel_residual'] <= 1e-8 * (1 + 1e-6), ml
+    assert ml['rel_diff_x'] <= 1e-6 and ml['iters'] <= 20 and ml['fallback_at'] is None, ml
     assert res['vertices'] == 2049 * 2049
     assert jac['true_rel_residual'] <= 1e-8 * (1 + 1e-6), jac
     assert mg['true_rel_residual'] <= 1e-8 * (1 + 1e-6), mg
