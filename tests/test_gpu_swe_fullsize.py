@@ -65,3 +65,18 @@ def test_full_size_lake_at_rest(gpu, full):
     h, uh, vh = eqn.h.collect(), eqn.uh.collect(), eqn.vh.collect()
     assert float((h - 1.0).abs().max()) <= 1e-12
     assert float(uh.abs().max()) <= 1e-10 and float(vh.abs().max()) <= 1e-10
+
+
+def test_full_size_float32_step_matches_oracle(gpu, full):
+    mesh, geo, dt = full
+    print(f'swe full size {N}x{N} float32 step vs oracle', flush=True)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=G, dtype=torch.float32)
+    esr.compile([eqn], 'hip')
+    ref = oswe.SWE(mesh, geo, dt=dt, g=G, dtype=torch.float32)
+    for _ in range(2):
+        eqn()
+        ref()
+    for k in ('h', 'uh', 'vh'):
+        got, want = getattr(eqn, k).collect(), getattr(ref, k)
+        scale = float(want.abs().max())
+        assert float((got - want).abs().max()) <= 1e-5 * scale, k
