@@ -98,9 +98,22 @@ class Unsupported(Exception):
 #                  8/7/7/6 waves by default; N = 8 left the first three at
 #                  256/318/318 us and made the fourth spill 44 bytes per lane
 #                  (292 -> 417 us, step 1.18 -> 1.31 ms), so the default is 0.
+#   remat        - an elementwise value that later kernels only gather (an RK
+#                  stage input h + c * d) is not written to a buffer: each
+#                  gather evaluates it at the source row from the arrays it
+#                  is made of (at most two, each stored anyway: tensors not
+#                  reassigned before the gather, and values other kernels
+#                  read), so one array write and one array read become reads
+#                  of arrays the gathering kernel mostly reads at its own rows.
+#   late_eval    - an elementwise value whose users all run in later kernels
+#                  is evaluated in the first of them instead of its earliest
+#                  kernel when that lowers the forward's algorithmic bytes
+#                  (e.g. d1 + d2 in the kernel that reads d2 anyway, saving
+#                  the partial sum's write and read); greedy, program order.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 16,
-           'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0}
+           'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0,
+           'remat': True, 'late_eval': True}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -365,17 +378,19 @@ class Program:
             raise Unsupported('no float tensors')
 
     # -- scheduling -----------------------------------------------------------
-    def schedule(self):
+    def schedule(self, late: Optional[Dict[int, int]] = None):
         """Assign every value the stage of the kernel that evaluates it.
 
         c(leaf) = -1 (ready before the forward); c(gather) = 0 for a leaf
         source, else c(source) + 1 (the source must be complete); c(ew/reduce)
-        = max(0, c(args)). Edge-space values are re-evaluated inside each
+        = max(0, c(args), late[v]) -- `late` holds evaluation stages chosen
+        later than the earliest one. Edge-space values are re-evaluated inside each
         reduce loop that uses them, so their evaluation stages are those of
         their consumers. A store runs no earlier than its value and after
         every evaluation that reads the stored tensor's old contents (strictly
         after, for reads at other rows through a Selector)."""
         sa = self.sa
+        late = late or {}
         self.edge_spaces = {sa.root(sa.red_vars[r][0]) for r in sa.red_vars}
         for v in self.values:
             if v.kind in ('leaf', 'scalar', 'const'):
@@ -384,7 +399,7 @@ class Program:
                 src = v.args[0]
                 v.stage = 0 if src.kind in ('leaf', 'scalar') else src.stage + 1
             elif v.kind in ('ew', 'reduce'):
-                v.stage = max([0] + [a.stage for a in v.args])
+                v.stage = max([0, late.get(v.id, 0)] + [a.stage for a in v.args])
         for st in self.stores:
             if st.space in self.edge_spaces:
                 raise Unsupported('assignment to an edge tensor')
@@ -565,32 +580,104 @@ class FusedEngine:
         return None
 
 
+class _DryBuf:
+    """Stands in for an intermediate buffer while candidate plans are only
+    costed (size and a unique, aligned address; never launched)."""
+    _next = 1 << 40
+
+    def __init__(self, n, itemsize):
+        self.n, self.itemsize = n, itemsize
+        _DryBuf._next += 1 << 12
+        self.ptr = _DryBuf._next
+
+    def data_ptr(self):
+        return self.ptr
+
+    def numel(self):
+        return self.n
+
+    def element_size(self):
+        return self.itemsize
+
+
+def _other_kernel_use(x: V, u: V) -> bool:
+    """u reads x outside x's own kernel row loop, not through a Selector."""
+    return u.kind != 'gather' and (u.stage != x.stage or u.space not in (x.space, None))
+
+
 class _Lowering:
-    def __init__(self, p: Program, prog: CompiledProgram):
+    def __init__(self, p: Program, prog: CompiledProgram, dry: bool = False):
         self.p, self.prog = p, prog
         self.ctype = 'double' if p.dtype == torch.float64 else 'float'
         self.device = prog.device
         self.kernels: List[Kernel] = []
         self.no_stage = False
         self.force_rpt1 = False
+        self.dry = dry
+        self.remat: set = set()
 
     def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
         a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
         return a is not None and b is not None and a.shape == b.shape and \
             red.easier_perm is None and np.array_equal(a, b)
 
+    def _remat_set(self) -> set:
+        """Ids of gathered elementwise values that their gathers recompute
+        (OPTIONS['remat']). The value's expression is followed through
+        elementwise nodes of its space down to constants, scalars, tensors
+        (original contents) and values stored for another kernel anyway; at
+        most two such arrays, and no tensor among them may be reassigned in
+        or before the last kernel that gathers (its old contents are read at
+        other rows)."""
+        p = self.p
+        out = set()
+        for v in p.values:
+            if v.kind != 'ew' or v.space is None or v.space in p.edge_spaces:
+                continue
+            gusers = [u for u in v.users if u.kind == 'gather']
+            if not gusers or any(_other_kernel_use(v, u) for u in v.users):
+                continue
+            last = max(max(p.evals.get(u.id, {u.stage})) for u in gusers)
+            arrays, tensors, seen, stack, ok = set(), set(), set(), [v], True
+            while stack and ok:
+                x = stack.pop()
+                if x.id in seen:
+                    continue
+                seen.add(x.id)
+                if x.kind in ('const', 'scalar'):
+                    continue
+                if x.kind == 'leaf' and x.space == v.space:
+                    arrays.add(x.id)
+                    tensors.add(x.obj)
+                elif x is not v and x.space == v.space and x.kind in ('ew', 'reduce') and \
+                        any(_other_kernel_use(x, u) for u in x.users):
+                    arrays.add(x.id)
+                elif x.kind == 'ew' and x.space in (v.space, None):
+                    stack.extend(x.args)
+                else:
+                    ok = False
+            if not ok or len(arrays) > 2:
+                continue
+            if any(st.obj in tensors and st.stage <= last for st in p.stores):
+                continue
+            out.add(v.id)
+        return out
+
     def materialize(self):
         p = self.p
+        self.remat = self._remat_set() if OPTIONS['remat'] else set()
         for v in p.values:
             if v.kind not in ('ew', 'reduce', 'gather') or v.space is None or \
                     v.space in p.edge_spaces:
                 continue
-            need = any(u.kind == 'gather' or u.stage != v.stage or
-                       u.space not in (v.space, None) for u in v.users)
+            need = any(_other_kernel_use(v, u) or
+                       (u.kind == 'gather' and v.id not in self.remat) for u in v.users)
             v.mat = need
+            v.buf, v.pack = None, None
             if need:
                 n = self.prog.space_len[v.space]
-                v.buf = torch.empty(n, dtype=p.dtype, device=self.device)
+                v.buf = _DryBuf(n, p.dtype.itemsize) if self.dry else \
+                    torch.empty(n, dtype=p.dtype, device=self.device)
         if OPTIONS['pack_intermediates'] and OPTIONS['rows_per_thread'] == 1:
             # the values one kernel materialises share one row-interleaved
             # buffer: row r of value j at r * width + j
@@ -602,7 +689,8 @@ class _Lowering:
                 if len(vs) < 2:
                     continue
                 n = self.prog.space_len[space]
-                buf = torch.empty(n * len(vs), dtype=p.dtype, device=self.device)
+                buf = _DryBuf(n * len(vs), p.dtype.itemsize) if self.dry else \
+                    torch.empty(n * len(vs), dtype=p.dtype, device=self.device)
                 for j, v in enumerate(vs):
                     v.buf, v.pack = buf, (len(vs), j)
 
@@ -741,6 +829,27 @@ class _Lowering:
         def is_pred(v):
             return v.kind == 'ew' and v.op in _PRED
 
+        def emit_at(y: V, gi: str, stmts: List[str], memo: Dict[tuple, str]) -> str:
+            """y at row `gi` of its space, y a value its gathers recompute:
+            its elementwise expression over the arrays it is made of, read
+            at that row (same operations, same order)."""
+            key = (y.id, gi)
+            if key in memo:
+                return memo[key]
+            if y.kind == 'const':
+                s = _fmt_const(y.const, ct)
+            elif y.kind == 'scalar':
+                s = emit_row(y)
+            elif y.kind == 'leaf' or y.mat:
+                s = var(stmts, f'{buf_of(y)}[{elem(y, gi)}]')
+            elif y.kind == 'ew':
+                a = [emit_at(x, gi, stmts, memo) for x in y.args]
+                s = var(stmts, ew_expr(y, a), 'bool' if is_pred(y) else None)
+            else:
+                raise Unsupported(f'{y.kind} in a recomputed gather source')
+            memo[key] = s
+            return s
+
         def emit_row(v: V) -> str:
             if v.id in cx.memo:
                 return cx.memo[v.id]
@@ -757,7 +866,11 @@ class _Lowering:
                 sel = v.obj
                 src = v.args[0]
                 idx = self._param(kern, sel.idx, 'int')
-                s = var(cx.rows, f'{buf_of(src)}[{elem(src, f"{idx}[{cx.R}]")}]')
+                if src.id in self.remat:
+                    gi = var(cx.rows, f'{idx}[{cx.R}]', 'int')
+                    s = emit_at(src, gi, cx.rows, {})
+                else:
+                    s = var(cx.rows, f'{buf_of(src)}[{elem(src, f"{idx}[{cx.R}]")}]')
             elif v.kind == 'reduce':
                 emit_group(v)
                 s = cx.memo[v.id]
@@ -797,6 +910,7 @@ class _Lowering:
                 self._param(kern, red.easier_perm, 'int')
             body: List[str] = []
             emem: Dict[int, str] = {}
+            amem: Dict[tuple, str] = {}
             gid = len(cx.groups)
             # edge arrays staged through LDS: param name -> (lds name, ctype)
             staged: Dict[str, tuple] = {}
@@ -847,6 +961,8 @@ class _Lowering:
                     if self._rowlocal(x.obj, red):
                         s = emit_row(src) if src.kind != 'leaf' else \
                             emit_row_leaf(src)
+                    elif src.id in self.remat:
+                        s = emit_at(src, gather_index(x.obj, src), body, amem)
                     else:
                         s = var(body, f'{buf_of(src)}[{elem(src, gather_index(x.obj, src))}]')
                 elif x.kind == 'ew':
@@ -1054,11 +1170,48 @@ def lower_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
     """(program, lowering, source) of a forward; raises Unsupported."""
     p = Program(m, g, prog)
     p.run()
-    p.schedule()
+    late = _late_stages(p, prog) if OPTIONS['late_eval'] else {}
+    p.schedule(late)
     low = _Lowering(p, prog)
     low.materialize()
     src = low.build()
     return p, low, src
+
+
+def _plan_bytes(p: Program, prog: CompiledProgram, late: Dict[int, int]) -> int:
+    """Algorithmic bytes of all kernels of the plan with evaluation stages
+    `late` (buffers stand-ins, nothing allocated or compiled)."""
+    p.schedule(late)
+    low = _Lowering(p, prog, dry=True)
+    low.materialize()
+    low.build()
+    return sum(k.algorithmic_bytes() for k in low.kernels)
+
+
+def _late_stages(p: Program, prog: CompiledProgram) -> Dict[int, int]:
+    """OPTIONS['late_eval']: in program order, each elementwise row value
+    whose users are all elementwise values of its space in later kernels is
+    moved to the first of those kernels when the plan's bytes drop."""
+    late: Dict[int, int] = {}
+    best = _plan_bytes(p, prog, late)
+    for x in list(p.values):
+        if x.kind != 'ew' or x.space is None or x.space in p.edge_spaces or not x.users:
+            continue
+        if any(u.kind != 'ew' or u.space != x.space for u in x.users):
+            continue
+        target = min(u.stage for u in x.users)
+        if target <= x.stage:
+            continue
+        trial = dict(late)
+        trial[x.id] = target
+        try:
+            b = _plan_bytes(p, prog, trial)
+        except Unsupported:
+            b = None
+        if b is not None and b < best:
+            late, best = trial, b
+        p.schedule(late)
+    return late
 
 
 def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):

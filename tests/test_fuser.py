@@ -37,8 +37,16 @@ def test_swe_step_fuses_into_four_kernels(dtype, kind):
     p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
     # one kernel per RK4 stage: the stage inputs are gathered by the next one
     assert [k.stage for k in low.kernels] == [0, 1, 2, 3]
-    # stage inputs U1..U3 and stage deltas d1..d3 (3 fields each) are written
-    assert sum(v.mat for v in p.values) == 18
+    # written per field: the stage deltas d1, d2, the partial sum
+    # (d1 + d2) + d3 and the last stage input U3 = h + dt d3; U1 and U2 are
+    # recomputed by the gathers of the next stage from h and d1 / d2, and
+    # U3 is not, as the last kernel reassigns h
+    assert sum(v.mat for v in p.values) == 12
+    assert len(low.remat) == 6
+    nc = prog.space_len[low.kernels[0].space]
+    cell_arrays = [sum(len(k.access[n]) for n, _, t in k.params if t.numel() == nc)
+                   for k in low.kernels]
+    assert cell_arrays == [7, 10, 16, 13]
     assert [st.stage for st in p.stores] == [3, 3, 3]
     ctype = 'double' if dtype == torch.float64 else 'float'
     assert f'const {ctype}' in src
@@ -173,4 +181,61 @@ def test_linsys_apply_fuses():
     idx = [i for i, mm in enumerate(prog.modules) if mm is m][0]
     p, low, src = fuser.lower_module(prog.modules[idx], prog.graphs[idx], prog)
     assert len(low.kernels) == 1
+    _rtc_ok(src)
+
+
+@pytest.mark.parametrize('remat,late_eval,mat,arrays', [
+    (False, False, 18, [10, 16, 16, 13]), (True, False, 15, [7, 13, 16, 13]),
+    (False, True, 18, [10, 16, 16, 13]), (True, True, 12, [7, 10, 16, 13])])
+def test_recomputed_gathers_and_late_evaluation(remat, late_eval, mat, arrays):
+    """Each option alone and together on the SWE step: values written and
+    cell arrays read or written per kernel (the algorithmic bytes only fall)."""
+    mesh = wmesh.tri2(6)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81)
+    prog = compile_program([eqn], 'rcm', torch.device('cpu'))
+    saved = dict(fuser.OPTIONS)
+    try:
+        fuser.OPTIONS.update(remat=remat, late_eval=late_eval)
+        p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    finally:
+        fuser.OPTIONS.update(saved)
+    nc = prog.space_len[low.kernels[0].space]
+    assert sum(v.mat for v in p.values) == mat
+    assert [sum(len(k.access[n]) for n, _, t in k.params if t.numel() == nc)
+            for k in low.kernels] == arrays
+    _rtc_ok(src)
+
+
+def test_recompute_refused_when_the_kernel_reassigns_the_source():
+    """A gathered value over a tensor that the gathering kernel reassigns is
+    written to a buffer (its gathers read the old contents at other rows);
+    over tensors left alone, the gathers recompute it."""
+    n = 40
+
+    class Two(esr.Module):
+        def __init__(self):
+            super().__init__()
+            g = torch.Generator().manual_seed(3)
+            self.sel = esr.Selector(torch.randint(0, n, (3 * n,), generator=g))
+            self.red = esr.Reducer(torch.arange(3 * n) % n, n)
+            self.u = esr.Tensor(torch.rand(n, dtype=torch.float64, generator=g),
+                                mode='partition')
+            self.v = esr.Tensor(torch.rand(n, dtype=torch.float64, generator=g),
+                                mode='partition')
+            self.w = esr.Tensor(torch.rand(n, dtype=torch.float64, generator=g),
+                                mode='partition')
+            self.x = esr.Tensor(torch.zeros(n, dtype=torch.float64), mode='partition')
+
+        def forward(self):
+            self.x[:] = self.red(self.sel(self.w * 2 + self.v))
+            self.u[:] = self.red(self.sel(self.u * 3))
+
+    m = Two()
+    prog = compile_program([m], 'rcm', torch.device('cpu'))
+    p, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    gathered = [u.args[0] for u in p.values if u.kind == 'gather']
+    a, b = gathered
+    assert a.id in low.remat and not a.mat
+    assert b.id not in low.remat and b.mat
     _rtc_ok(src)

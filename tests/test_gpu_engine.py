@@ -188,12 +188,14 @@ def test_swe_steps_match_oracle(gpu, dtype, gen, g, fuse):
 
 
 @pytest.mark.parametrize('option', [('rows_per_thread', 2), ('pack_intermediates', True),
-                                    ('waves_per_eu', 8)])
+                                    ('waves_per_eu', 8), ('remat', False),
+                                    ('late_eval', False)])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_codegen_options_are_bit_identical(gpu, dtype, option):
     """Fused kernels generated with two rows per thread (vector row
-    accesses), with row-interleaved intermediate buffers or with an occupancy
-    request give the same bits as the defaults: the SWE step (even cell count) and a program on an odd
+    accesses), with row-interleaved intermediate buffers, with an occupancy
+    request, with gathered stage inputs written instead of recomputed, or with
+    every value in its earliest kernel give the same bits as the defaults: the SWE step (even cell count) and a program on an odd
     number of rows (the last thread's second row is past the end)."""
     from easier_amd.core import fuser
     key, val = option
