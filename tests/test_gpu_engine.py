@@ -313,3 +313,47 @@ def test_mixed_precision_cg_matches_fp64_oracle(gpu):
     r = b - A(x)
     torch.testing.assert_close(torch.tensor(info['residual'], dtype=torch.float64), r.norm(),
                                rtol=1e-6, atol=0)
+
+
+def test_recomputed_gathers_match_the_node_engine(gpu):
+    """Gathered elementwise values recomputed at their source rows (random
+    Selector indices, so neighbours lie anywhere), a gathered value over a
+    tensor its gathering kernel reassigns (written and read back instead),
+    and a partial sum evaluated in a later kernel: the fused kernels give the
+    node-by-node engine's bits over several forwards."""
+    from easier_amd.core import fuser
+    n = 3001
+
+    class Prog(esr.Module):
+        def __init__(self):
+            super().__init__()
+            g = torch.Generator().manual_seed(11)
+            self.sel = esr.Selector(torch.randint(0, n, (3 * n,), generator=g))
+            self.red = esr.Reducer(torch.arange(3 * n) % n, n)
+            mk = lambda: esr.Tensor(torch.rand(n, dtype=torch.float64, generator=g),
+                                    mode='partition')
+            self.u, self.v, self.w = mk(), mk(), mk()
+            self.c = esr.Tensor(torch.rand(3 * n, dtype=torch.float64, generator=g),
+                                mode='partition')
+            self.x = esr.Tensor(torch.zeros(n, dtype=torch.float64), mode='partition')
+
+        def forward(self):
+            d1 = self.red(self.sel(self.w * 0.5 + self.v) * self.c)
+            d2 = self.red(self.sel(self.w + 0.25 * d1))
+            d3 = self.red(self.sel(self.w - 0.125 * d2))
+            self.x[:] = self.x + (d1 + d2 + d3) * 0.01
+            self.u[:] = self.red(self.sel(self.u * 3)) * 0.25
+
+    outs = []
+    for fuse in (True, False):
+        m = Prog()
+        print(f'recomputed gathers, fuse={fuse}', flush=True)
+        esr.compile([m], 'hip', fuse=fuse)
+        eng = m.easier_engine
+        if fuse:
+            assert isinstance(eng, fuser.FusedEngine)
+        for _ in range(4):
+            m()
+        outs.append((m.x.collect(), m.u.collect()))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
