@@ -617,9 +617,15 @@ class _Lowering:
         self.remat: set = set()
 
     def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
+        """The Selector's index equals the Reducer's (it reads the row
+        itself); computed once per pair, as every candidate plan asks."""
         a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
-        return a is not None and b is not None and a.shape == b.shape and \
-            red.easier_perm is None and np.array_equal(a, b)
+        cache = sel.__dict__.setdefault('easier_rowlocal', {})
+        key = (id(red), id(a), id(b), red.easier_perm is None)
+        if key not in cache:
+            cache[key] = a is not None and b is not None and a.shape == b.shape and \
+                red.easier_perm is None and np.array_equal(a, b)
+        return cache[key]
 
     def _remat_set(self) -> set:
         """Ids of gathered elementwise values that their gathers recompute
