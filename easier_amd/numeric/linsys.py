@@ -1,3 +1,6 @@
+# Portions of this file reproduce EASIER (jack-pan-ai/EASIER),
+# Copyright (c) Microsoft Corporation, MIT License; the licence text is in
+# easier_amd/THIRD_PARTY_NOTICES.txt.
 """Linear-system operator (easier/numeric/linsys.py:10-24)."""
 from torch import nn
 

@@ -1,3 +1,6 @@
+# Portions of this file reproduce EASIER (jack-pan-ai/EASIER),
+# Copyright (c) Microsoft Corporation, MIT License; the licence text is in
+# easier_amd/THIRD_PARTY_NOTICES.txt.
 """Preconditioned conjugate gradients (easier/numeric/solver/cg.py:12-137).
 
 Init/Step/Update are the reference's three esr.Modules with the same

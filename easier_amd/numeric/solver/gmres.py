@@ -1,3 +1,6 @@
+# Portions of this file reproduce EASIER (jack-pan-ai/EASIER),
+# Copyright (c) Microsoft Corporation, MIT License; the licence text is in
+# easier_amd/THIRD_PARTY_NOTICES.txt.
 """Restarted, right-preconditioned GMRES with modified Gram-Schmidt
 (easier/numeric/solver/gmres.py:12-266).
 

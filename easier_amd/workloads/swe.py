@@ -1,3 +1,6 @@
+# Portions of this file reproduce EASIER (jack-pan-ai/EASIER),
+# Copyright (c) Microsoft Corporation, MIT License; the licence text is in
+# easier_amd/THIRD_PARTY_NOTICES.txt.
 """The shallow-water tutorial programs on in-memory meshes.
 
 ShallowWaterMeshComponentsCollector and ShallowWaterInitializer follow

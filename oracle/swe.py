@@ -1,3 +1,6 @@
+# Portions of this file reproduce EASIER (jack-pan-ai/EASIER),
+# Copyright (c) Microsoft Corporation, MIT License; the licence text is in
+# easier_amd/THIRD_PARTY_NOTICES.txt.
 """Shallow-water geometry and RK4 step restated with torch CPU operators.
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
