@@ -53,8 +53,7 @@ def build_swe(n, dtype, seed, fuse=True):
     mesh = wmesh.tri2(n)
     log(f'mesh {n}x{n}: {mesh["cells"].shape[0]} cells, {mesh["src"].shape[0]} '
         f'edges ({time.perf_counter() - t0:.1f} s)')
-    rng = np.random.Generator(np.random.PCG64(seed))
-    x0 = tuple(float(v) for v in rng.uniform(0.3, 0.7, 2))
+    x0 = wswe.gaussian_centre(seed)
     t0 = time.perf_counter()
     geo = wswe.assemble_geometry(mesh, centre=x0, width=1.0 / 0.005, amp=0.1)
     log(f'geometry programs on the GPU ({time.perf_counter() - t0:.1f} s)')

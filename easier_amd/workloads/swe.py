@@ -259,6 +259,14 @@ class ShallowWaterEquation(esr.Module):
             delta_vh1 + delta_vh2 + delta_vh3 + delta_vh4)
 
 
+def gaussian_centre(seed):
+    """Centre x0 of the initial Gaussian, uniform in [0.3, 0.7]^2 from numpy's
+    PCG64 seeded with `seed`."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return tuple(float(v) for v in rng.uniform(0.3, 0.7, 2))
+
+
 def cfl_dt(mesh, h0, g, cfl=0.4):
     """dt = cfl * L_min / (max|u| + sqrt(g * h_max)) at rest (u = 0), with
     L_min the shortest mesh edge."""
