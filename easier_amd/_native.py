@@ -33,6 +33,7 @@ _DTYPE_CODES = {
 RED_OPS = {'sum': 0, 'prod': 1, 'mean': 2, 'amax': 3, 'amin': 4}
 AGG_SUM, AGG_PROD, AGG_NORM, AGG_MAX, AGG_MIN = range(5)
 SEQ_ROW_MAX = 256
+AGG_SCRATCH_BYTES = 131072   # ESR_AGG_SCRATCH_BYTES
 
 # elementwise op codes (esr_hip.h ESR_EW_*)
 EW = dict(
@@ -60,6 +61,8 @@ _SIGNATURES = {
     'esr_aggregate_ws_bytes': (_c_i32, [_c_i32, _c_i64, _c_i64, _P_i64]),
     'esr_aggregate': (_c_i32, [_c_i32, _c_i32, _c_vp, _c_i64, _c_i64, _c_dbl,
                                _c_vp, _c_vp, _c_i64, _c_vp]),
+    'esr_aggregate_noalloc': (_c_i32, [_c_i32, _c_i32, _c_vp, _c_i64, _c_i64,
+                                       _c_dbl, _c_vp, _c_vp]),
     'esr_ewise': (_c_i32, [_c_i32, _c_i32, _c_vp, _c_i32, _c_vp, _c_vp,
                            _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_fill': (_c_i32, [_c_i32, _c_vp, _c_i64, _c_dbl, _c_vp]),

@@ -7,6 +7,7 @@
 // tiles on one XCD and its L2.
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <type_traits>
 
 #include "esr_common.h"
@@ -800,6 +801,55 @@ ESR_API int esr_aggregate(int op, int dtype, const void *x, int64_t n, int64_t w
     set_error("esr_aggregate: unsupported dtype %d", dtype);
     return ESR_ERR_UNSUPPORTED;
   }
+  ESR_GUARD_END
+}
+
+// ----------------------------------------------------------------------------
+// esr_aggregate_noalloc: the same two stages with the stage-1 partials in a
+// device array the library holds (one copy per GPU, for the life of the
+// process), for callers that cannot allocate.
+// ----------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[ESR_AGG_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address; the
+// array's name in host code is only a stand-in for the lookup).
+int scratch_on_current_device(void **out) {
+  int dev = -1;
+  ESR_HIP(hipGetDevice(&dev));
+  ESR_REQUIRE(dev >= 0 && dev < kMaxDevices, "esr_aggregate_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void *p = nullptr;
+    ESR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ESR_REQUIRE(p != nullptr, "esr_aggregate_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return ESR_OK;
+}
+} // namespace
+
+ESR_API int esr_aggregate_noalloc(int op, int dtype, const void *x, int64_t n, int64_t width,
+                                  double p, void *out, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 0 && width >= 1, "esr_aggregate_noalloc: bad sizes");
+  ESR_REQUIRE(dtype_size(dtype) > 0, "esr_aggregate_noalloc: unknown dtype %d", dtype);
+  int64_t need = 0;
+  esr_aggregate_ws_bytes(dtype, n, width, &need);
+  ESR_REQUIRE(need <= (int64_t)ESR_AGG_SCRATCH_BYTES,
+              "esr_aggregate_noalloc: %lld x %lld values need %lld scratch bytes, more than the "
+              "%d the library holds (use esr_aggregate with a workspace)",
+              (long long)n, (long long)width, (long long)need, (int)ESR_AGG_SCRATCH_BYTES);
+  void *ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc != ESR_OK) return rc;
+  return esr_aggregate(op, dtype, x, n, width, p, out, ws, need, stream);
   ESR_GUARD_END
 }
 

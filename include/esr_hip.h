@@ -10,7 +10,8 @@
  *   - all device buffers are owned by the caller (plain device pointers and
  *     sizes, no framework types); the library allocates device memory only
  *     inside objects it hands back (compiled modules, graph executables), each
- *     with an explicit destroy call;
+ *     with an explicit destroy call, plus one fixed device array per GPU for
+ *     esr_aggregate_noalloc (ESR_AGG_SCRATCH_BYTES, held until exit);
  *   - every device launch goes on the caller's stream (`stream` is a
  *     hipStream_t passed as void*; NULL means the null stream);
  *   - index arrays are int32 on the device; their ranges are validated by the
@@ -97,6 +98,20 @@ int esr_aggregate_ws_bytes(int dtype, int64_t n, int64_t width, int64_t *bytes);
 int esr_aggregate(int op, int dtype, const void *x, int64_t n, int64_t width,
                   double p, void *out, void *ws, int64_t ws_bytes,
                   void *stream);
+/* esr_aggregate without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The stage-1 partials go to a device array of
+ * ESR_AGG_SCRATCH_BYTES bytes that the library holds on every GPU it runs on
+ * until the process ends; the device address is looked up on the first call
+ * on each device. Requests whose workspace (esr_aggregate_ws_bytes) exceeds
+ * that size are rejected with ESR_ERR_INVALID (f64: any n for width <= 16).
+ * Every call on one GPU shares the array, so calls that overlap in time on
+ * one GPU (two streams, two host threads) race; calls ordered on one stream
+ * do not. Results are bit-identical to esr_aggregate's.                     */
+#define ESR_AGG_SCRATCH_BYTES 131072
+int esr_aggregate_noalloc(int op, int dtype, const void *x, int64_t n,
+                          int64_t width, double p, void *out, void *stream);
 
 /* ---- elementwise operators of a traced esr.Module forward ----------------
  * Executes one torch elementwise operator (the call_function/call_method
