@@ -9,7 +9,12 @@ dt from CFL 0.4. The program is the tutorial's ShallowWaterEquation module
 compiled with esr.compile(..., 'hip'); a step is one forward() (one RK4 step).
 
 Prints one JSON line (rank 0, N = 1). Side figures: the P1 Poisson
-Jacobi-PCG solve of the 100 x 100 mesh (10,201 vertices, tol 1e-8).
+Jacobi-PCG solve of the 100 x 100 mesh (10,201 vertices, tol 1e-8), the
+primitives, the float32 step and, by default (--configs 4,5), the
+variable-coefficient 2048 x 2048 Poisson solves: float64 and mixed-precision
+Jacobi-PCG and multigrid flexible CG to 1e-10, and Jacobi-PCG against the
+multigrid and learned-multigrid flexible CG to 1e-8 (the metric's solve-time
+half).
 """
 import argparse
 import json
@@ -70,7 +75,11 @@ def build_swe(n, dtype, seed, fuse=True):
 
 def time_kernels(eqn, reps):
     """Average duration of each fused kernel, from HIP events on the stream
-    the kernels are launched on (the current stream)."""
+    the kernels are launched on (the current stream): `reps` back-to-back
+    launches of the one kernel between two events, so no event sits between
+    launches. (A stage kernel relaunched reads the arrays it reads inside
+    the step; the last one updates the state in place, which leaves the
+    state advanced but the work per launch unchanged.)"""
     import ctypes
     import torch
     from easier_amd import _native as nat
@@ -79,36 +88,88 @@ def time_kernels(eqn, reps):
         return None
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-    totals = [0.0] * len(eng.launches)
-    for _ in range(reps):
-        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(eng.launches) + 1)]
-        evs[0].record(stream)
-        for k, (fn, grid, buf, block) in enumerate(eng.launches):
+    out = []
+    for fn, grid, buf, block in eng.launches:
+        nat.call('esr_launch', fn, grid, 1, 1, block, 1, 1, 0, sp,
+                 ctypes.addressof(buf), ctypes.sizeof(buf))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
             nat.call('esr_launch', fn, grid, 1, 1, block, 1, 1, 0, sp,
                      ctypes.addressof(buf), ctypes.sizeof(buf))
-            evs[k + 1].record(stream)
+        b.record(stream)
         torch.cuda.synchronize()
-        for k in range(len(eng.launches)):
-            totals[k] += evs[k].elapsed_time(evs[k + 1]) / 1e3
-    return [t / reps for t in totals]
+        out.append(a.elapsed_time(b) / 1e3 / reps)
+    return out
 
 
-def cpu_baseline_swe(mesh, geo, meta, dtype, steps):
+def host_cpu():
+    """(model name from /proc/cpuinfo, CPUs this process may run on)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def cpu_baseline_swe(mesh, geo, meta, dtype, steps, samples=3):
     """The oracle (torch CPU operators of the reference's CPU backend) on the
-    same mesh and state for a bounded number of RK4 steps."""
+    same mesh and state: after one warm-up step, the median of `samples`
+    timed runs of `steps` RK4 steps, on every CPU this process may use."""
     import torch
     from oracle import swe as oswe
+    model, cores = host_cpu()
+    torch.set_num_threads(cores)
     ref = oswe.SWE(mesh, geo, dt=meta['dt'], g=meta['g'], dtype=dtype)
     ref()                                   # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ref()
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ref()
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[len(times) // 2]
     nc = mesh['cells'].shape[0]
     return dict(value=nc * steps / dt, unit='cells/s', cores=torch.get_num_threads(),
-                kind='port', sample=f'{steps} RK4 steps of the same {nc}-cell '
-                f'mesh after 1 warm-up step, torch CPU operators',
-                seconds=dt)
+                kind='port', sample=f'median of {samples} runs of {steps} RK4 steps of '
+                f'the same {nc}-cell mesh after 1 warm-up step, torch CPU operators',
+                seconds=dt, all_seconds=times, cpu_model=model,
+                os_cpu_count=os.cpu_count())
+
+
+def pcg_iteration_roofline(cg, n, nnz, T, chunks=3):
+    """One Jacobi-PCG iteration of a fused CG, timed with HIP events on the
+    launch stream around `chunks` replays of its captured chunk graph
+    (32 iterations each) of a fresh solve armed with tolerance 0, so that
+    every iteration runs. Algorithmic bytes of an iteration:
+    nnz (I + T) + (n + 1) I + 14 n T (the operator once, 14 vector passes)."""
+    import torch
+    f = cg.easier_fused
+    cg.x.data.zero_()
+    f.arm(0.0, None, None)
+    stream = torch.cuda.current_stream()
+    f.graph.replay()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(chunks):
+        f.graph.replay()
+    b.record(stream)
+    torch.cuda.synchronize()
+    it = a.elapsed_time(b) / 1e3 / (chunks * f.graph_chunk)
+    bytes_ = nnz * (4 + T) + (n + 1) * 4 + 14 * n * T
+    return dict(bound='hbm', kernel='Jacobi-PCG iteration (k_pcg_step1/2/3 of one step)',
+                achieved=bytes_ / it / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                frac=bytes_ / it / HBM_PEAK, traffic=None, bytes_per_launch=bytes_,
+                avg_iteration_s=it)
 
 
 def poisson_config1(skip_cpu):
@@ -393,13 +454,18 @@ def mixed_config4(reps=1, skip_cpu=False):
     t64, i64 = _best_time(lambda: cg.solve(rtol=1e-10, maxiter=100000), reps,
                           reset=lambda: e1.x.data.zero_())
     x64 = e1.x.collect().numpy()
+    roof = pcg_iteration_roofline(cg, int(sys_['Av'].shape[0]), int(sys_['row'].shape[0]), 8)
+    log(f'config 4 fp64 Jacobi-PCG iteration: {roof["avg_iteration_s"] * 1e6:.1f} us, '
+        f'{roof["frac"]:.1%} of 8 TB/s')
     e2 = wp.PoissonP1(sys_)
     mp = MixedPrecisionCG(e2.A, e2.b, e2.x, e2.M)
     esr.compile([e2, mp], 'hip')
     tmx, imx = _best_time(lambda: mp.solve(rtol=1e-10, inner_rtol=1e-4, maxiter=100000),
                           reps, reset=lambda: e2.x.data.zero_())
     xmx = e2.x.collect().numpy()
-    out = dict(vertices=int(sys_['Av'].shape[0]), fp64_s=t64, fp64_iters=i64['iters'],
+    out = dict(roofline=roof, fp64_s_per_iter=t64 / max(i64['iters'] + 1, 1),
+               mixed_s_per_inner_iter=tmx / max(imx['iters'], 1),
+               vertices=int(sys_['Av'].shape[0]), fp64_s=t64, fp64_iters=i64['iters'],
                fp64_residual=i64['residual'], mixed_s=tmx, mixed_inner_iters=imx['iters'],
                mixed_outer=imx['outer'], mixed_true_residual=imx['residual'],
                bnorm=imx['bnorm'],
@@ -448,6 +514,7 @@ def learned_config5(weights, seeds=(1,), reps=1,
     for s in seeds:
         res = wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000,
                                    methods=methods)
+        res['network_application'] = wl.time_forward(pre, 2048, s, reps=20)
         if not skip_cpu and 'jacobi' in res:
             rate = cpu_pcg_rate(wp.varcoef_system(2048, s)[1])
             res['cpu_jacobi_s_per_iter'] = rate
@@ -495,9 +562,10 @@ def main():
     ap.add_argument('--no-fuse', action='store_true')
     ap.add_argument('--profile', action='store_true',
                     help='only warm-up + timed steps (for rocprofv3 runs)')
-    ap.add_argument('--configs', default='',
+    ap.add_argument('--configs', default='4,5',
                     help='comma list of extra workloads measured after the headline: '
-                         '3 (batched PCG), 4 (mixed precision), 5 (learned FCG)')
+                         '3 (batched PCG), 4 (mixed precision), 5 (learned FCG); '
+                         'empty for none')
     ap.add_argument('--weights', default=os.path.join(REPO, 'easier_amd', 'workloads',
                                                       'gnn_smoother_l3_h32.npz'))
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -584,7 +652,6 @@ def main():
     if not args.skip_cpu:
         log(f'CPU baseline: {args.cpu_steps} oracle steps on the host')
         result['cpu_baseline'] = cpu_baseline_swe(mesh, geo, meta, dtype, args.cpu_steps)
-        result['cpu_baseline']['cpu_model'] = platform.processor() or platform.machine()
     if not args.skip_poisson:
         log('Poisson config 1')
         result['poisson_config1'] = poisson_config1(args.skip_cpu)
@@ -601,7 +668,21 @@ def main():
     if '4' in extra:
         result['mixed_config4'] = mixed_config4(skip_cpu=args.skip_cpu)
     if '5' in extra:
+        log('config 5: Jacobi-PCG, multigrid and learned-multigrid flexible CG, 2048^2')
         result['learned_config5'] = learned_config5(args.weights, skip_cpu=args.skip_cpu)
+        r5 = result['learned_config5'][0]
+        if 'jacobi' in r5 and 'mg-learned' in r5:
+            result['poisson_solve_2048'] = {
+                'workload': 'variable-coefficient P1 Poisson 2048x2048 (4,198,401 '
+                            'vertices), seed 1, FP64, rtol 1e-8, from x = 0',
+                'jacobi_pcg_s': r5['jacobi']['seconds'],
+                'jacobi_pcg_iters': r5['jacobi']['iters'],
+                'learned_mg_fcg_s': r5['mg-learned']['seconds'],
+                'learned_mg_fcg_iters': r5['mg-learned']['iters'],
+                'learned_speedup_vs_jacobi': r5['mg-learned']['speedup_vs_jacobi'],
+                'mg_fcg_s': r5.get('mg', {}).get('seconds'),
+                'mg_fcg_iters': r5.get('mg', {}).get('iters'),
+            }
     print(json.dumps(result), flush=True)
 
 

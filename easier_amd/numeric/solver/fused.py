@@ -171,19 +171,25 @@ class FusedCG:
         return dict(rnorm=sc[0], bnorm=sc[1], rz=sc[2], tol=sc[3], iters=it[0],
                     done=it[1], k=it[2])
 
-    def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None,
-              chunk=None):
+    def arm(self, rtol, atol=None, maxiter=None, chunk=None):
+        """Init (cg.py:25-29) and the exit tolerance on the device; the
+        chunk graph is captured on first use."""
         c = self.cg
-        name = c.__class__.__name__
         nat.call('esr_pcg_init', self.dt, self.op.n, *self._arrays(),
                  c.b.data.data_ptr(), c.x.data.data_ptr(), c.r.data.data_ptr(),
                  c.z.data.data_ptr(), c.p.data.data_ptr(), self.state.data_ptr(),
                  self.ws.data_ptr(), float(rtol), float(atol) if atol else 0.0,
                  -1 if maxiter is None else int(maxiter), nat.stream_ptr())
-        chunk = chunk or (debug_iter if debug_iter else self.CHUNK)
+        chunk = chunk or self.CHUNK
         if self.graph is None or self.graph_chunk != chunk:
             self.graph = _Graph(lambda sp: self.steps(chunk, sp))
             self.graph_chunk = chunk
+
+    def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None,
+              chunk=None):
+        c = self.cg
+        name = c.__class__.__name__
+        self.arm(rtol, atol, maxiter, chunk or (debug_iter if debug_iter else None))
         while True:
             self.graph.replay()
             s = self.read()
