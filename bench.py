@@ -6,7 +6,10 @@ Workload (default): the 2-triangle mesh of a 2048 x 2048 cell grid
 faces), state (h, hu, hv) in float64, h = 1 + 0.1 exp(-|x - x0|^2 / 0.005)
 with x0 ~ U[0.3, 0.7]^2 (seed 0), hu = hv = 0, g = 9.81, reflective walls,
 dt from CFL 0.4. The program is the tutorial's ShallowWaterEquation module
-compiled with esr.compile(..., 'hip'); a step is one forward() (one RK4 step).
+compiled with esr.compile(..., 'hip', freeze=True) (the geometry, which no
+forward writes, may be read through bit-exact dictionaries; --no-freeze
+reads it as plain arrays, and the line carries that step time too); a step
+is one forward() (one RK4 step).
 
 Prints one JSON line (rank 0, N = 1). Side figures: the P1 Poisson
 Jacobi-PCG solve of the 100 x 100 mesh (10,201 vertices, tol 1e-8), the
@@ -48,7 +51,7 @@ def swe_bytes(nc, ne, nbc, T, I=4):
     return stages, sum(stages)
 
 
-def build_swe(n, dtype, seed, fuse=True):
+def build_swe(n, dtype, seed, fuse=True, freeze=False):
     import numpy as np
     import torch
     import easier_amd as esr
@@ -66,7 +69,7 @@ def build_swe(n, dtype, seed, fuse=True):
     dt = wswe.cfl_dt(mesh, geo['h'], g, cfl=0.4)
     t0 = time.perf_counter()
     eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=g, dtype=dtype)
-    esr.compile([eqn], 'hip', fuse=fuse)
+    esr.compile([eqn], 'hip', fuse=fuse, freeze=freeze)
     torch.cuda.synchronize()
     log(f'compiled ({type(eqn.easier_engine).__name__}, '
         f'{time.perf_counter() - t0:.1f} s), dt = {dt:.3e}')
@@ -323,10 +326,10 @@ def poisson_gmres_config1(skip_cpu):
     return out
 
 
-def swe_fp32(n, seed, steps=100, warmup=5):
-    """The same SWE step in float32 (cells/s; parity 1e-5 in the tests)."""
+def swe_side(n, seed, dtype, freeze, steps=100, warmup=5):
+    """The SWE step in another precision or compile mode (cells/s)."""
     import torch
-    mesh, geo, eqn, meta = build_swe(n, torch.float32, seed)
+    mesh, geo, eqn, meta = build_swe(n, dtype, seed, freeze=freeze)
     for _ in range(warmup):
         eqn()
     torch.cuda.synchronize()
@@ -336,16 +339,16 @@ def swe_fp32(n, seed, steps=100, warmup=5):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     nc = mesh['cells'].shape[0]
-    return dict(workload=f'shallow-water RK4 {n}x{n}, float32', steps=steps,
+    return dict(workload=f'shallow-water RK4 {n}x{n}, {dtype}, freeze={freeze}', steps=steps,
                 ms_per_step=1e3 * wall / steps, cells_per_s=nc * steps / wall,
                 state_finite=bool(torch.isfinite(eqn.h.collect()).all()))
 
 
-def committed_traffic(n, dtype, kernel):
+def committed_traffic(n, dtype, kernel, tag=''):
     """HBM bytes per launch of `kernel` measured by the rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes summarised in profiles/ (lines 'traffic_bytes <kernel>
     <bytes>'), or None when no summary covers this workload."""
-    path = os.path.join(REPO, 'profiles', f'swe_fused_{n}_{dtype}_traffic.md')
+    path = os.path.join(REPO, 'profiles', f'swe_fused_{n}_{dtype}{tag}_traffic.md')
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -560,6 +563,9 @@ def main():
     ap.add_argument('--skip-fp32', action='store_true',
                     help='skip the float32 SWE side figure')
     ap.add_argument('--no-fuse', action='store_true')
+    ap.add_argument('--no-freeze', action='store_true',
+                    help='compile without freeze=True (geometry read as plain arrays '
+                         'instead of through dictionaries)')
     ap.add_argument('--profile', action='store_true',
                     help='only warm-up + timed steps (for rocprofv3 runs)')
     ap.add_argument('--configs', default='4,5',
@@ -585,7 +591,9 @@ def main():
         return
     dtype = torch.float64 if args.dtype == 'f64' else torch.float32
     T = 8 if dtype == torch.float64 else 4
-    mesh, geo, eqn, meta = build_swe(args.n, dtype, args.seed, fuse=not args.no_fuse)
+    freeze = not args.no_freeze
+    mesh, geo, eqn, meta = build_swe(args.n, dtype, args.seed, fuse=not args.no_fuse,
+                                     freeze=freeze)
     nc, ne, nbc = mesh['cells'].shape[0], mesh['src'].shape[0], mesh['bcells'].shape[0]
     for _ in range(args.warmup):
         eqn()
@@ -622,7 +630,10 @@ def main():
             'workload': f'shallow-water RK4, 2-triangle mesh {args.n}x{args.n} '
                         f'({nc} cells, {ne} directed edges, {nbc} wall faces), '
                         f'{args.dtype}, g=9.81, CFL 0.4 (dt={meta["dt"]:.4e}), '
-                        'reflective walls',
+                        'reflective walls' + (', compiled with freeze=True (constant '
+                                              'geometry read through dictionaries)'
+                                              if freeze else ''),
+            'freeze': freeze,
             'cells': nc, 'edges': ne, 'engine': type(eqn.easier_engine).__name__,
             'step_bytes_algorithmic': step_bytes,
             'step_bytes_formula': swe_bytes(nc, ne, nbc, T)[1],
@@ -642,7 +653,8 @@ def main():
             'bound': 'hbm', 'kernel': f'esr_fused_{k} (RK4 stage {k + 1} of 4)',
             'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
             'frac': achieved / HBM_PEAK,
-            'traffic': committed_traffic(args.n, args.dtype, f'esr_fused_{k}'),
+            'traffic': committed_traffic(args.n, args.dtype, f'esr_fused_{k}',
+                                         '_frozen' if freeze else ''),
             'bytes_per_launch': stages[k], 'avg_launch_s': ktimes[k],
             'all_kernels_s': ktimes,
         }
@@ -661,7 +673,12 @@ def main():
         log('SWE float32')
         del eqn
         torch.cuda.empty_cache()
-        result['swe_fp32'] = swe_fp32(args.n, args.seed)
+        result['swe_fp32'] = swe_side(args.n, args.seed, torch.float32, freeze)
+        if freeze:
+            log('SWE float64 without freeze')
+            torch.cuda.empty_cache()
+            result['swe_fp64_unfrozen'] = swe_side(args.n, args.seed, torch.float64, False,
+                                                   steps=40)
     extra = {c.strip() for c in args.configs.split(',') if c.strip()}
     if '3' in extra:
         result['batched_config3'] = batched_config3()

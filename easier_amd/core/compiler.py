@@ -327,6 +327,7 @@ class CompiledProgram:
         self.spaces: Optional[SpaceAnalysis] = None
         self.device: Optional[torch.device] = None
         self.bound_reducer: Dict[int, _M.Reducer] = {}
+        self.freeze = False
 
 
 def _validate_idx(name, idx: np.ndarray, upper: Optional[int], kind: str):

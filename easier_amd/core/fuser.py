@@ -110,10 +110,31 @@ class Unsupported(Exception):
 #                  kernel when that lowers the forward's algorithmic bytes
 #                  (e.g. d1 + d2 in the kernel that reads d2 anyway, saving
 #                  the partial sum's write and read); greedy, program order.
+#   dict_max     - with esr.compile(..., freeze=True): a partition tensor no
+#                  forward of the program writes, whose values take at most
+#                  dict_max distinct bit patterns, is read through a
+#                  dictionary: one uint8 code per row (shared by all such
+#                  tensors of one index space when their tuples of values take
+#                  at most dict_max patterns together) and a table of values
+#                  staged in LDS; a tensor with one value becomes a kernel
+#                  constant read once. Bit-exact (the table holds the
+#                  tensor's own values); tensors under dict_min_rows rows are
+#                  left alone.
+#   ell          - with freeze=True: a dense Reducer group (at most 8 edges per
+#                  row, no permutation) reads its edge data from padded
+#                  per-row records (ELL: S = 4 or 8 slots per row, slot k of
+#                  row r at r S + k, empty slots after the row's edges) built
+#                  at compile time -- the int16 source offsets, the dictionary
+#                  codes and any other edge array -- each loaded with one
+#                  vector load per row, and the row's edge count (uint8)
+#                  instead of two row pointers: no row-pointer load ahead of
+#                  the edge loads, one instruction per array per row. Slots
+#                  fold in the same order, empty slots are skipped.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 16,
            'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0,
-           'remat': True, 'late_eval': True}
+           'remat': True, 'late_eval': True, 'dict_max': 256, 'dict_min_rows': 4096,
+           'ell': True}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -159,6 +180,179 @@ def _sparse_rows(red):
            torch.from_numpy(crow.astype(np.int32)).to(dev))
     red.easier_sparse_rows = out
     return out
+
+
+class _Enc:
+    """A frozen tensor read through a dictionary: codes (uint8 per row, or
+    None when the tensor holds one value) index lut (its distinct values)."""
+
+    def __init__(self, codes, lut):
+        self.codes, self.lut = codes, lut
+        self.k = int(lut.numel())
+
+
+def _written_tensors(prog: CompiledProgram) -> set:
+    """ids of the esr.Tensors some forward of the program may write: assigned
+    (setitem), updated in place (a method or function ending in '_'), an
+    `out=` target, or handed to a module call other than a Selector or
+    Reducer (through views taken with getitem)."""
+    out = set()
+    for m, g in zip(prog.modules, prog.graphs):
+        env = {}
+        for node in g.nodes:
+            if node.op == 'get_attr':
+                try:
+                    obj = resolve_attr(m, node.target)
+                except Exception:
+                    continue
+                if isinstance(obj, _M.Tensor):
+                    env[node] = obj
+
+        def mark(a, depth=0):
+            if not isinstance(a, fx.Node) or depth > 16:
+                return
+            if a in env:
+                out.add(id(env[a]))
+            elif a.op in ('call_function', 'call_method') and a.args and (
+                    a.target is operator.getitem or
+                    a.target in ('view', 'reshape', 'view_as', 'expand', 'squeeze',
+                                 'unsqueeze', 'flatten', 'narrow', 'select', '__getitem__')):
+                mark(a.args[0], depth + 1)
+        for node in g.nodes:
+            name = node.target if node.op == 'call_method' else \
+                getattr(node.target, '__name__', '')
+            if node.op == 'call_function' and node.target is operator.setitem:
+                mark(node.args[0])
+            elif node.op in ('call_method', 'call_function') and isinstance(name, str) \
+                    and name.endswith('_') and not name.endswith('__') and node.args:
+                mark(node.args[0])
+            if 'out' in node.kwargs:
+                mark(node.kwargs['out'])
+            if node.op == 'call_module':
+                sub = m.get_submodule(node.target)
+                if not isinstance(sub, (_M.Selector, _M.Reducer)):
+                    for a in node.args:
+                        mark(a)
+    return out
+
+
+def _encode_group(ts: list, prog: CompiledProgram) -> Dict[int, _Enc]:
+    """Dictionaries of the frozen tensors `ts` of one index space (once per
+    compile): tensors with one value become constants; the others share one
+    code array when their value tuples take at most dict_max patterns, else
+    each tensor with at most dict_max values gets its own."""
+    cache = prog.__dict__.setdefault('easier_dict_cache', {})
+    key = tuple(id(t) for t in ts)
+    if key in cache:
+        return cache[key]
+    maxk = int(OPTIONS['dict_max'])
+    ftype = ts[0].dtype
+    itype = np.int64 if ftype == torch.float64 else np.int32
+    dev = ts[0].data.device
+    per = []
+    for t in ts:
+        h = t.data.detach().cpu().numpy().view(itype)
+        if np.unique(h[::max(1, h.size // 65536)]).size > maxk:
+            per.append(None)
+            continue
+        u, inv = np.unique(h, return_inverse=True)
+        per.append(None if u.size > maxk else (u, inv.reshape(-1)))
+
+    def lut_of(vals):
+        return torch.from_numpy(np.ascontiguousarray(vals).view(
+            np.float64 if ftype == torch.float64 else np.float32).copy()).to(dev)
+    res: Dict[int, _Enc] = {}
+    multi = []
+    for i, x in enumerate(per):
+        if x is None:
+            continue
+        if x[0].size == 1:
+            res[id(ts[i])] = _Enc(None, lut_of(x[0]))
+        else:
+            multi.append(i)
+    multi = multi[:7]                      # 256^7 patterns fit an int64 key
+    joint = False
+    if len(multi) >= 2:
+        jk = np.zeros(per[multi[0]][1].shape[0], np.int64)
+        for i in multi:
+            jk = jk * maxk + per[i][1]
+        ju, jinv = np.unique(jk, return_inverse=True)
+        if ju.size <= maxk:
+            codes = torch.from_numpy(jinv.reshape(-1).astype(np.uint8)).to(dev)
+            rem = ju.copy()
+            for i in reversed(multi):
+                res[id(ts[i])] = _Enc(codes, lut_of(per[i][0][rem % maxk]))
+                rem //= maxk
+            joint = True
+    if not joint:
+        for i in multi:
+            u, inv = per[i]
+            res[id(ts[i])] = _Enc(torch.from_numpy(inv.astype(np.uint8)).to(dev), lut_of(u))
+    cache[key] = res
+    return res
+
+
+def _frozen_encoding(p: 'Program', prog: CompiledProgram) -> Dict[int, _Enc]:
+    """OPTIONS['dict_max'] under esr.compile(..., freeze=True): dictionaries
+    of the program's constant partition tensors, by id."""
+    written = _written_tensors(prog)
+    groups: Dict[object, list] = {}
+    for v in p.values:
+        if v.kind != 'leaf' or id(v.obj) in written:
+            continue
+        t = v.obj
+        if not t.is_partition or t.data.dim() != 1 or \
+                t.dtype not in (torch.float32, torch.float64) or \
+                t.data.numel() < int(OPTIONS['dict_min_rows']):
+            continue
+        lst = groups.setdefault(v.space, [])
+        if all(x is not t for x in lst):
+            lst.append(t)
+    out: Dict[int, _Enc] = {}
+    for ts in groups.values():
+        out.update(_encode_group(ts, prog))
+    return out
+
+
+def _ell_array(prog: CompiledProgram, red, S: int, key, host_values: np.ndarray, dev):
+    """An edge array of a Reducer-ordered edge space in the padded per-row
+    layout (row r's k-th edge at r * S + k, zeros in empty slots), once per
+    compile."""
+    cache = prog.__dict__.setdefault('easier_ell_cache', {})
+    ck = ('arr', key, id(red), S)
+    if ck not in cache:
+        slot = _ell_slots(prog, red, S)
+        out = np.zeros(slot.shape, host_values.dtype)
+        ok = slot >= 0
+        out[ok] = host_values[slot[ok]]
+        cache[ck] = torch.from_numpy(out.reshape(-1)).to(dev)
+    return cache[ck]
+
+
+def _ell_slots(prog: CompiledProgram, red, S: int) -> np.ndarray:
+    """(n, S) edge positions of each row's slots, -1 when empty."""
+    cache = prog.__dict__.setdefault('easier_ell_cache', {})
+    ck = ('slots', id(red), S)
+    if ck not in cache:
+        hp = np.asarray(red.easier_host_rowptr, np.int64)
+        n = hp.shape[0] - 1
+        counts = np.diff(hp)
+        ne = int(hp[-1])
+        rows = np.repeat(np.arange(n, dtype=np.int64), counts)
+        k = np.arange(ne, dtype=np.int64) - np.repeat(hp[:-1], counts)
+        slot = np.full((n, S), -1, np.int64)
+        slot[rows, k] = np.arange(ne, dtype=np.int64)
+        cache[ck] = slot
+    return cache[ck]
+
+
+def _ell_counts(prog: CompiledProgram, red, dev):
+    cache = prog.__dict__.setdefault('easier_ell_cache', {})
+    ck = ('cnt', id(red))
+    if ck not in cache:
+        counts = np.diff(np.asarray(red.easier_host_rowptr, np.int64))
+        cache[ck] = torch.from_numpy(counts.astype(np.uint8)).to(dev)
+    return cache[ck]
 
 
 _EW_BIN = {
@@ -496,6 +690,17 @@ __device__ __forceinline__ void esr_st2nt(float *p, i64 r, i64 n, float x, float
     __builtin_nontemporal_store(v, reinterpret_cast<esr_v2f *>(p + r));
   } else __builtin_nontemporal_store(x, p + r);
 }
+// per-row records of padded (ELL) edge arrays
+typedef short esr_s4 __attribute__((ext_vector_type(4)));
+typedef short esr_s8 __attribute__((ext_vector_type(8)));
+typedef unsigned char esr_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned char esr_u8 __attribute__((ext_vector_type(8)));
+typedef int esr_i4 __attribute__((ext_vector_type(4)));
+typedef int esr_i8 __attribute__((ext_vector_type(8)));
+typedef double esr_d4 __attribute__((ext_vector_type(4)));
+typedef double esr_d8 __attribute__((ext_vector_type(8)));
+typedef float esr_f4 __attribute__((ext_vector_type(4)));
+typedef float esr_f8 __attribute__((ext_vector_type(8)));
 // correctly rounded sqrt: device sqrt, then the Tuckerman rounding test
 __device__ __forceinline__ double esr_sqrt(double x) {
   const double y = sqrt(x);
@@ -540,6 +745,8 @@ class Kernel:
         self.lds_bytes = 0
         self.block = int(OPTIONS['block'])
         self.rpt = 1                      # rows per thread
+        self.luts: Dict[str, tuple] = {}  # dictionary param -> (LDS name, size)
+        self.consts: Dict[str, str] = {}  # one-value dictionary param -> name
 
     def algorithmic_bytes(self) -> int:
         """Compulsory HBM bytes of one launch: every array the kernel reads
@@ -615,6 +822,7 @@ class _Lowering:
         self.force_rpt1 = False
         self.dry = dry
         self.remat: set = set()
+        self.enc: Dict[int, _Enc] = {}
 
     def _rowlocal(self, sel: _M.Selector, red: _M.Reducer) -> bool:
         """The Selector's index equals the Reducer's (it reads the row
@@ -672,6 +880,8 @@ class _Lowering:
     def materialize(self):
         p = self.p
         self.remat = self._remat_set() if OPTIONS['remat'] else set()
+        self.enc = _frozen_encoding(p, self.prog) if \
+            getattr(self.prog, 'freeze', False) and OPTIONS['dict_max'] else {}
         for v in p.values:
             if v.kind not in ('ew', 'reduce', 'gather') or v.space is None or \
                     v.space in p.edge_spaces:
@@ -798,6 +1008,35 @@ class _Lowering:
             stmts.append(f'  const {typ or ct} {name} = {expr};')
             return name
         memo_names: List[str] = []
+        code_memo: Dict[tuple, str] = {}
+
+        def leaf_at(t, pos, stmts):
+            """Frozen tensor t at position `pos` through its dictionary (a
+            kernel constant, or the LDS table at the row's code); None when t
+            is read as an array."""
+            enc = self.enc.get(id(t))
+            if enc is None:
+                return None
+            lut = lut_name(enc)
+            if enc.codes is None:
+                return lut
+            cp = self._param(kern, enc.codes, 'unsigned char')
+            key = (cp, str(pos), id(stmts))
+            if key not in code_memo:
+                code_memo[key] = var(stmts, f'(int){cp}[{pos}]', 'int')
+            return f'{lut}[{code_memo[key]}]'
+
+        def lut_name(enc):
+            """The kernel's name of a dictionary: its LDS table, or the
+            constant of a one-value dictionary."""
+            lut = self._param(kern, enc.lut, ct)
+            if enc.codes is None:
+                if lut not in kern.consts:
+                    kern.consts[lut] = f'c{len(kern.consts)}'
+                return kern.consts[lut]
+            if lut not in kern.luts:
+                kern.luts[lut] = (f'L{len(kern.luts)}', enc.k)
+            return kern.luts[lut][0]
 
         def ew_expr(v, a):
             op = v.op
@@ -846,6 +1085,8 @@ class _Lowering:
                 s = _fmt_const(y.const, ct)
             elif y.kind == 'scalar':
                 s = emit_row(y)
+            elif y.kind == 'leaf' and y.obj is not None and id(y.obj) in self.enc:
+                s = var(stmts, leaf_at(y.obj, gi, stmts))
             elif y.kind == 'leaf' or y.mat:
                 s = var(stmts, f'{buf_of(y)}[{elem(y, gi)}]')
             elif y.kind == 'ew':
@@ -863,6 +1104,10 @@ class _Lowering:
                 s = _fmt_const(v.const, ct)
             elif v.kind == 'scalar':
                 s = var(cx.rows, f'{buf_of(v)}[0]')
+            elif v.kind == 'leaf' and cx.comp is None and id(v.obj) in self.enc:
+                if v.space != kern.space:
+                    raise Unsupported('row read from another space')
+                s = var(cx.rows, leaf_at(v.obj, cx.R, cx.rows))
             elif v.kind == 'leaf' or (v.mat and v.stage < kern.stage) or \
                     (v.mat and v.space != kern.space):
                 if v.space != kern.space:
@@ -905,11 +1150,19 @@ class _Lowering:
             # non-empty rows replaces the row pointers
             sparse = (not dense and OPTIONS['sparse_rows'] and counts.size >= 4096 and
                       np.count_nonzero(counts) * 8 <= counts.size)
+            ell = (dense and red.easier_perm is None and kmax <= 8 and cx.comp is None and
+                   getattr(self.prog, 'freeze', False) and OPTIONS['ell'] and
+                   not (OPTIONS['stage_edges'] and not self.no_stage))
+            S = 4 if kmax <= 4 else 8
             if sparse:
                 bm, pre, crow = _sparse_rows(red)
                 bmp = self._param(kern, bm, 'unsigned long long')
                 prp = self._param(kern, pre, 'int')
                 rp = self._param(kern, crow, 'int')
+            elif ell:
+                rp = None
+                cntp = self._param(kern, _ell_counts(self.prog, red, self.device),
+                                   'unsigned char')
             else:
                 rp = self._param(kern, red.easier_rowptr, 'int')
             perm = None if red.easier_perm is None else \
@@ -923,6 +1176,33 @@ class _Lowering:
             stage = dense and perm is None and OPTIONS['stage_edges'] and not self.no_stage
             nt = OPTIONS['nt_loads']
             B = kern.block
+
+            ellv: Dict[str, str] = {}
+            tags = {'short': 's', 'unsigned char': 'u', 'int': 'i', 'double': 'd',
+                    'float': 'f'}
+
+            def ell_vec(t, ctype):
+                """Row record of a padded edge array, loaded once per row
+                (at row scope); slot k is <name>[k]."""
+                name = self._param(kern, t, ctype)
+                if name not in ellv:
+                    ellv[name] = f'E{gid}_{len(ellv)}'
+                    cx.rows.append(f'  const esr_{tags[ctype]}{S} {ellv[name]} = '
+                                   f'*(const esr_{tags[ctype]}{S} *)({name} + (i64){cx.R} * {S});')
+                return ellv[name]
+
+            def ell_leaf(t) -> str:
+                """Edge tensor t at slot k of the row."""
+                enc = self.enc.get(id(t))
+                if enc is not None and enc.codes is None:
+                    return lut_name(enc)
+                if enc is not None:
+                    codes = _ell_array(self.prog, red, S, ('codes', id(enc.codes)),
+                                       enc.codes.cpu().numpy(), self.device)
+                    return f'{lut_name(enc)}[(int){ell_vec(codes, "unsigned char")}[k]]'
+                raw = _ell_array(self.prog, red, S, ('raw', id(t)),
+                                 t.data.detach().cpu().numpy(), self.device)
+                return f'{ell_vec(raw, ct)}[k]'
 
             def edge_load(name, ctype):
                 if stage:
@@ -942,6 +1222,21 @@ class _Lowering:
                 clamped slots of an empty row may belong to another row."""
                 if id(sel) in gidx:
                     return gidx[id(sel)]
+                hi = self.prog.space_len[src.space] - 1
+                if ell:
+                    d = _delta16(sel, red) if OPTIONS['idx16'] else None
+                    if d is not None:
+                        o = _ell_array(self.prog, red, S, ('d16', id(sel)),
+                                       d.cpu().numpy(), self.device)
+                        s = var(body, f'esr_clampi((int){cx.R} + (int){ell_vec(o, "short")}[k], '
+                                f'{hi})', 'int')
+                    else:
+                        ix = _ell_array(self.prog, red, S, ('idx', id(sel)),
+                                        np.asarray(sel.easier_host_idx).astype(np.int32),
+                                        self.device)
+                        s = var(body, f'esr_clampi({ell_vec(ix, "int")}[k], {hi})', 'int')
+                    gidx[id(sel)] = s
+                    return s
                 d = _delta16(sel, red) if (dense and perm is None and not stage and
                                            OPTIONS['idx16']) else None
                 if d is not None:
@@ -960,6 +1255,10 @@ class _Lowering:
                     return emem[x.id]
                 if x.space != v.args[0].space:
                     s = emit_row(x)           # row value or constant, hoisted
+                elif x.kind == 'leaf' and ell:
+                    s = var(body, ell_leaf(x.obj))
+                elif x.kind == 'leaf' and not stage and id(x.obj) in self.enc:
+                    s = var(body, leaf_at(x.obj, 'e', body))
                 elif x.kind == 'leaf':
                     s = var(body, edge_load(buf_of(x), ct))
                 elif x.kind == 'gather':
@@ -969,6 +1268,8 @@ class _Lowering:
                             emit_row_leaf(src)
                     elif src.id in self.remat:
                         s = emit_at(src, gather_index(x.obj, src), body, amem)
+                    elif src.kind == 'leaf' and id(src.obj) in self.enc:
+                        s = var(body, leaf_at(src.obj, gather_index(x.obj, src), body))
                     else:
                         s = var(body, f'{buf_of(src)}[{elem(src, gather_index(x.obj, src))}]')
                 elif x.kind == 'ew':
@@ -1011,7 +1312,15 @@ class _Lowering:
                 cx.rows.append(f'  if (nl{gid} == 0 && threadIdx.x == 0) {{ ' +
                             ' '.join(f'{sn}[0] = 0;' for sn, _ in staged.values()) + ' }')
                 cx.rows.append('  __syncthreads();')
-            if dense:
+            if ell:
+                for u, a in zip(members, accs):
+                    body.append(f'  {a} = (k < cn{gid}) ? {a} + {inner[members.index(u)]} : {a};')
+                cx.rows.append(f'  {{ const int cn{gid} = (int){cntp}[{cx.R}];')
+                cx.rows.append(f'  #pragma unroll')
+                cx.rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
+                cx.rows.extend('  ' + b for b in body)
+                cx.rows.append('  } }')
+            elif dense:
                 # every row has at most kmax entries: evaluate kmax slots with
                 # clamped (always valid) edge positions so all their loads
                 # issue together, and fold slot k into the sums only when it
@@ -1135,6 +1444,18 @@ class _Lowering:
             head = (f'  const i64 r = {tile} * {kern.block} + threadIdx.x;\n'
                     '  if (r >= a.n) return;\n')
             tail = '\n'.join(st)
+        # dictionary tables to LDS (every thread takes part, before any
+        # thread leaves), one-value dictionaries as constants
+        pre = []
+        for lname, (ln, k) in kern.luts.items():
+            pre.append(f'  __shared__ {ct} {ln}[{k}];')
+            pre.append(f'  for (int i = threadIdx.x; i < {k}; i += {kern.block}) '
+                       f'{ln}[i] = {lname}[i];')
+        for lname, cn in kern.consts.items():
+            pre.append(f'  const {ct} {cn} = {lname}[0];')
+        if kern.luts:
+            pre.append('  __syncthreads();')
+        head = ''.join(x + '\n' for x in pre) + head
         wpe = (f' __attribute__((amdgpu_waves_per_eu({int(OPTIONS["waves_per_eu"])})))'
                if OPTIONS['waves_per_eu'] else '')
         kern.src = (

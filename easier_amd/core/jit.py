@@ -65,7 +65,8 @@ def _validate_compile_args(top_modules, backend, partition_mode):
 
 def compile(modules: List[_M.Module], backend: str = 'hip', *,
             load_dir: Optional[str] = None, partition_mode: str = 'metis',
-            reorder: str = 'rcm', fuse: bool = True) -> List[_M.Module]:
+            reorder: str = 'rcm', fuse: bool = True,
+            freeze: bool = False) -> List[_M.Module]:
     """Compile esr.Modules for one MI355X.
 
     - partition_mode: accepted for compatibility; one GPU holds one partition.
@@ -76,6 +77,11 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
       every node runs as its own primitive kernel.
     - load_dir: a directory written by esr.dump(); its renumbering is reused
       when it matches this program (core/cache.py), else compiled afresh.
+    - freeze: the caller promises that partition tensors no compiled forward
+      writes (mesh geometry, coefficients) keep their values after compile();
+      fused kernels may then read such a tensor through a compile-time
+      dictionary of its distinct values (bit-exact; see core/fuser.py). To
+      change one, compile again.
     """
     top_modules = _validate_compile_args(modules, backend, partition_mode)
     cache = None
@@ -90,6 +96,7 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
     logger.info(f"EASIER just-in-time compilation has started, "
                 f"backend={backend}, device={device}")
     prog = compile_program(top_modules, reorder, device, cache)
+    prog.freeze = bool(freeze)
     from .engine import HipEngine
     fused = None
     if fuse:

@@ -239,3 +239,58 @@ def test_recompute_refused_when_the_kernel_reassigns_the_source():
     assert a.id in low.remat and not a.mat
     assert b.id not in low.remat and b.mat
     _rtc_ok(src)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_frozen_constants_are_dictionary_encoded_bit_exactly(dtype):
+    """compile(..., freeze=True): the SWE geometry (alpha, sx, sy per edge,
+    area per cell) is read through dictionaries
+    whose tables reproduce every value bit for bit; the state (written by
+    the forward) is not encoded; the kernels compile and move fewer bytes."""
+    mesh = wmesh.tri2(64)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81, dtype=dtype)
+    prog = compile_program([eqn], 'rcm', torch.device('cpu'))
+    _, low0, _ = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    plain = [k.algorithmic_bytes() for k in low0.kernels]
+    prog.freeze = True
+    _, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    frozen = [k.algorithmic_bytes() for k in low.kernels]
+    names = {id(getattr(eqn, k)): k for k in ('alpha', 'sx', 'sy', 'bsx', 'bsy', 'area',
+                                                'h', 'uh', 'vh')}
+    encoded = {names[i] for i in low.enc}
+    # the 256 wall faces are under the size at which a dictionary pays
+    assert encoded == {'alpha', 'sx', 'sy', 'area'}
+    bits = torch.int64 if dtype == torch.float64 else torch.int32
+    for i, enc in low.enc.items():
+        t = [getattr(eqn, k) for k in names.values() if id(getattr(eqn, k)) == i][0].data
+        got = enc.lut[0].expand(t.shape) if enc.codes is None else enc.lut[enc.codes.long()]
+        assert torch.equal(got.view(bits), t.view(bits)), names[i]
+    # the edge arrays with several values share one code array (in float32
+    # alpha rounds to 1/2 everywhere and is a constant); area holds one value
+    codes = {id(low.enc[id(getattr(eqn, k))].codes) for k in ('alpha', 'sx', 'sy')
+             if low.enc[id(getattr(eqn, k))].codes is not None}
+    assert len(codes) == 1
+    assert low.enc[id(eqn.area)].codes is None
+    assert all(f < p for f, p in zip(frozen, plain)), (frozen, plain)
+    assert '__shared__' in src and '__syncthreads();' in src
+    _rtc_ok(src)
+
+
+def test_frozen_encoding_skips_written_and_many_valued_tensors():
+    """A tensor the forward assigns is not a constant; an edge array with
+    more distinct values than the dictionary holds is read as an array."""
+    mesh = wmesh.tri2(40, jitter=0.2, seed=1)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81, dtype=torch.float64)
+    prog = compile_program([eqn], 'rcm', torch.device('cpu'))
+    prog.freeze = True
+    _, low, src = fuser.lower_module(prog.modules[0], prog.graphs[0], prog)
+    enc_ids = set(low.enc)
+    for k in ('h', 'uh', 'vh', 'alpha', 'sx', 'sy', 'area'):
+        assert id(getattr(eqn, k)) not in enc_ids, k      # jittered: all distinct
+    assert np.unique(geo['alpha'].numpy()).size > 256
+    written = fuser._written_tensors(prog)
+    assert {id(eqn.h), id(eqn.uh), id(eqn.vh)} <= written
+    assert id(eqn.sx) not in written
+    _rtc_ok(src)

@@ -55,6 +55,23 @@ def test_full_size_step_bit_identical_and_mass_conserved(gpu, full):
     assert abs(m20 - m0) <= 1e-12 * abs(m0), (m0, m20)
 
 
+def test_full_size_frozen_step_bit_identical(gpu, full):
+    """compile(..., freeze=True) reads the geometry through dictionaries
+    (the uniform mesh's alpha, sx, sy take few values; area one): the same
+    bits as the oracle after two steps."""
+    mesh, geo, dt = full
+    print(f'swe full size {N}x{N} frozen steps vs oracle', flush=True)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=G, dtype=torch.float64)
+    esr.compile([eqn], 'hip', freeze=True)
+    assert any(k.luts for k in eqn.easier_engine.kernels)
+    ref = oswe.SWE(mesh, geo, dt=dt, g=G, dtype=torch.float64)
+    for _ in range(2):
+        eqn()
+        ref()
+    for k in ('h', 'uh', 'vh'):
+        assert torch.equal(getattr(eqn, k).collect(), getattr(ref, k)), k
+
+
 def test_full_size_lake_at_rest(gpu, full):
     mesh, geo, dt = full
     print(f'swe full size {N}x{N} lake at rest', flush=True)
