@@ -51,7 +51,7 @@ def swe_bytes(nc, ne, nbc, T, I=4):
     return stages, sum(stages)
 
 
-def build_swe(n, dtype, seed, fuse=True, freeze=False):
+def build_swe(n, dtype, seed, fuse=True, freeze=False, reorder='rcm'):
     import numpy as np
     import torch
     import easier_amd as esr
@@ -69,7 +69,7 @@ def build_swe(n, dtype, seed, fuse=True, freeze=False):
     dt = wswe.cfl_dt(mesh, geo['h'], g, cfl=0.4)
     t0 = time.perf_counter()
     eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=g, dtype=dtype)
-    esr.compile([eqn], 'hip', fuse=fuse, freeze=freeze)
+    esr.compile([eqn], 'hip', fuse=fuse, freeze=freeze, reorder=reorder)
     torch.cuda.synchronize()
     log(f'compiled ({type(eqn.easier_engine).__name__}, '
         f'{time.perf_counter() - t0:.1f} s), dt = {dt:.3e}')
@@ -329,7 +329,8 @@ def poisson_gmres_config1(skip_cpu):
 def swe_side(n, seed, dtype, freeze, steps=100, warmup=5):
     """The SWE step in another precision or compile mode (cells/s)."""
     import torch
-    mesh, geo, eqn, meta = build_swe(n, dtype, seed, freeze=freeze)
+    mesh, geo, eqn, meta = build_swe(n, dtype, seed, freeze=freeze,
+                                     reorder='blocks' if freeze else 'rcm')
     for _ in range(warmup):
         eqn()
     torch.cuda.synchronize()
@@ -563,6 +564,8 @@ def main():
     ap.add_argument('--skip-fp32', action='store_true',
                     help='skip the float32 SWE side figure')
     ap.add_argument('--no-fuse', action='store_true')
+    ap.add_argument('--reorder', choices=['rcm', 'blocks', 'none'], default=None,
+                    help="cell renumbering (default: 'blocks' with freeze, else 'rcm')")
     ap.add_argument('--no-freeze', action='store_true',
                     help='compile without freeze=True (geometry read as plain arrays '
                          'instead of through dictionaries)')
@@ -592,8 +595,9 @@ def main():
     dtype = torch.float64 if args.dtype == 'f64' else torch.float32
     T = 8 if dtype == torch.float64 else 4
     freeze = not args.no_freeze
+    reorder = args.reorder or ('blocks' if freeze else 'rcm')
     mesh, geo, eqn, meta = build_swe(args.n, dtype, args.seed, fuse=not args.no_fuse,
-                                     freeze=freeze)
+                                     freeze=freeze, reorder=reorder)
     nc, ne, nbc = mesh['cells'].shape[0], mesh['src'].shape[0], mesh['bcells'].shape[0]
     for _ in range(args.warmup):
         eqn()
@@ -631,9 +635,9 @@ def main():
                         f'({nc} cells, {ne} directed edges, {nbc} wall faces), '
                         f'{args.dtype}, g=9.81, CFL 0.4 (dt={meta["dt"]:.4e}), '
                         'reflective walls' + (', compiled with freeze=True (constant '
-                                              'geometry read through dictionaries)'
-                                              if freeze else ''),
-            'freeze': freeze,
+                                              'geometry read through dictionaries), '
+                                              f'reorder={reorder}' if freeze else ''),
+            'freeze': freeze, 'reorder': reorder,
             'cells': nc, 'edges': ne, 'engine': type(eqn.easier_engine).__name__,
             'step_bytes_algorithmic': step_bytes,
             'step_bytes_formula': swe_bytes(nc, ne, nbc, T)[1],

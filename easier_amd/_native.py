@@ -85,6 +85,7 @@ _SIGNATURES = {
     'esr_csr_from_idx': (_c_i32, [_c_vp, _c_i64, _c_i64, _c_vp, _c_vp]),
     'esr_rcm': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_i64, _c_vp]),
     'esr_bandwidth': (_c_i32, [_c_vp, _c_vp, _c_i64, _P_i64]),
+    'esr_block_order': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp]),
     'esr_p1_nnz': (_c_i32, [_c_i64, _c_i64, _c_vp, _P_i64]),
     'esr_p1_assemble': (_c_i32, [_c_i64, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp,
                                  _c_i64] + [_c_vp] * 6),

@@ -299,6 +299,18 @@ def rcm_order(n: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return order
 
 
+# rows per tile of the row kernels (their workgroup size) for reorder='blocks'
+BLOCK_ROWS = 256
+
+
+def block_order(n: int, a: np.ndarray, b: np.ndarray, block: int) -> np.ndarray:
+    a, b = _i64(a), _i64(b)
+    order = np.empty(n, dtype=np.int64)
+    nat.call('esr_block_order', n, a.ctypes.data if a.size else None,
+             b.ctypes.data if b.size else None, a.shape[0], int(block), order.ctypes.data)
+    return order
+
+
 def bandwidth(a: np.ndarray, b: np.ndarray) -> int:
     a, b = _i64(a), _i64(b)
     bw = ctypes.c_int64(0)
@@ -427,7 +439,7 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
     # node spaces: RCM over (reducer target, selector source) pairs
     if cached is not None:
         order = {root: o for root, (k, o) in cached.items() if k == 'rcm'}
-    elif reorder == 'rcm':
+    elif reorder in ('rcm', 'blocks'):
         pairs: Dict[int, list] = {}
         for r in reducers:
             for s in selectors:
@@ -440,6 +452,15 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
             a = np.concatenate([p[0] for p in plist])
             b = np.concatenate([p[1] for p in plist])
             n = space_len[space]
+            if reorder == 'blocks':
+                # tiles of BLOCK_ROWS rows (one workgroup of the row kernels)
+                # hold compact pieces of the graph
+                o = block_order(n, a, b, BLOCK_ROWS)
+                order[space] = o
+                inv = _inverse(o)
+                logger.debug(f'blocks on a space of {n}: edges within a tile '
+                             f'{np.mean(inv[a] // BLOCK_ROWS == inv[b] // BLOCK_ROWS):.3f}')
+                continue
             o = rcm_order(n, a, b)
             inv = _inverse(o)
             if bandwidth(inv[a], inv[b]) < bandwidth(a, b):
@@ -447,7 +468,8 @@ def compile_program(top_modules: List[_M.Module], reorder: str,
             logger.debug(f'RCM on a space of {n}: bandwidth '
                          f'{bandwidth(a, b)} -> {bandwidth(inv[a], inv[b])}')
     elif reorder != 'none':
-        raise EasierJitException(f"reorder must be 'rcm' or 'none', got {reorder!r}")
+        raise EasierJitException(
+            f"reorder must be 'rcm', 'blocks' or 'none', got {reorder!r}")
     for root in order:
         kind[root] = 'rcm'
 

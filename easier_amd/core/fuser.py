@@ -130,11 +130,20 @@ class Unsupported(Exception):
 #                  instead of two row pointers: no row-pointer load ahead of
 #                  the edge loads, one instruction per array per row. Slots
 #                  fold in the same order, empty slots are skipped.
+#   halo         - with ell: the values a group gathers through a Selector
+#                  are staged in LDS per tile of rows -- the tile's own rows
+#                  (each thread its row) and the tile's halo (the other source
+#                  rows its edges read, a list built at compile time, at most
+#                  halo_max per tile) -- and each edge slot reads LDS at a
+#                  uint16 index; one coalesced load and one halo load per
+#                  array and tile replace a gather per edge slot (the
+#                  gathers' address processing bounds the kernels otherwise).
+#                  Tiles whose halo does not fit keep the gathers.
 OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 256,
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 16,
            'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0,
            'remat': True, 'late_eval': True, 'dict_max': 256, 'dict_min_rows': 4096,
-           'ell': True}
+           'ell': True, 'dict_lds': True, 'persist': 0, 'halo': True, 'halo_max': 256}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -353,6 +362,61 @@ def _ell_counts(prog: CompiledProgram, red, dev):
         counts = np.diff(np.asarray(red.easier_host_rowptr, np.int64))
         cache[ck] = torch.from_numpy(counts.astype(np.uint8)).to(dev)
     return cache[ck]
+
+
+def _halo_layout(prog: CompiledProgram, sel, red, S: int, B: int, hmax: int, dev):
+    """Per tile of B rows: (local index per slot (uint16, n * S), halo rows
+    (int32, ntiles * HS), HS) for the Selector's source rows of a
+    Reducer-ordered edge space, or None when some tile reads more than hmax
+    rows outside itself. Slot k of row r reads LDS entry i: i < B is row
+    tile * B + i, i >= B the tile's halo row i - B."""
+    cache = prog.__dict__.setdefault('easier_ell_cache', {})
+    ck = ('halo', id(sel), id(red), S, B, hmax)
+    if ck in cache:
+        return cache[ck]
+    slot = _ell_slots(prog, red, S)
+    n = slot.shape[0]
+    idx = np.asarray(sel.easier_host_idx, np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64)[:, None], S, axis=1)
+    src = np.where(slot >= 0, idx[np.maximum(slot, 0)], rows)
+    tile = rows // B
+    loc = src - tile * B
+    ext = (loc < 0) | (loc >= B)
+    ntiles = (n + B - 1) // B
+    keys = tile[ext] * n + src[ext]
+    uk = np.unique(keys)
+    ut = uk // n
+    counts = np.bincount(ut, minlength=ntiles)
+    hs = int(counts.max()) if counts.size else 0
+    if hs > hmax:
+        cache[ck] = None
+        return None
+    hs = max(64, (hs + 63) // 64 * 64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    pos = np.arange(uk.size) - starts[ut]
+    halo = (np.arange(ntiles, dtype=np.int64)[:, None] * B).repeat(hs, 1)
+    halo[ut, pos] = uk % n
+    halo = np.minimum(halo, n - 1)
+    local = loc.copy()
+    local[ext] = B + pos[np.searchsorted(uk, keys)]
+    out = (torch.from_numpy(local.astype(np.uint16).reshape(-1)).to(dev),
+           torch.from_numpy(halo.astype(np.int32).reshape(-1)).to(dev), hs)
+    cache[ck] = out
+    return out
+
+
+class _LocalIdx(str):
+    """Name of a kernel variable holding an LDS-local gather index (halo
+    mode); `sel` identifies the Selector whose staged values it indexes."""
+
+    def __new__(cls, name, sel):
+        obj = str.__new__(cls, name)
+        obj.sel = sel
+        return obj
+
+
+class _NoHalo(Exception):
+    """A kernel's halo staging exceeds the LDS budget."""
 
 
 _EW_BIN = {
@@ -691,6 +755,8 @@ __device__ __forceinline__ void esr_st2nt(float *p, i64 r, i64 n, float x, float
   } else __builtin_nontemporal_store(x, p + r);
 }
 // per-row records of padded (ELL) edge arrays
+typedef unsigned short esr_w4 __attribute__((ext_vector_type(4)));
+typedef unsigned short esr_w8 __attribute__((ext_vector_type(8)));
 typedef short esr_s4 __attribute__((ext_vector_type(4)));
 typedef short esr_s8 __attribute__((ext_vector_type(8)));
 typedef unsigned char esr_u4 __attribute__((ext_vector_type(4)));
@@ -745,6 +811,7 @@ class Kernel:
         self.lds_bytes = 0
         self.block = int(OPTIONS['block'])
         self.rpt = 1                      # rows per thread
+        self.persist = 0                  # > 0: workgroups loop over row tiles
         self.luts: Dict[str, tuple] = {}  # dictionary param -> (LDS name, size)
         self.consts: Dict[str, str] = {}  # one-value dictionary param -> name
 
@@ -819,6 +886,7 @@ class _Lowering:
         self.device = prog.device
         self.kernels: List[Kernel] = []
         self.no_stage = False
+        self.no_halo = False
         self.force_rpt1 = False
         self.dry = dry
         self.remat: set = set()
@@ -930,6 +998,12 @@ class _Lowering:
                     kern.n = self.prog.space_len[space]
                     self._emit_kernel(kern, roots)
                     self.force_rpt1 = False
+            except _NoHalo:
+                self.no_halo = True
+                kern = Kernel(space, stage, f'esr_fused_{k}')
+                kern.n = self.prog.space_len[space]
+                self._emit_kernel(kern, roots)
+                self.no_halo = False
             except _Restage:
                 # too many staged edge arrays for the LDS budget: this
                 # kernel reads its edges straight from global memory
@@ -977,6 +1051,7 @@ class _Lowering:
             R = 'r'
             comp = None
             sfx = ''
+            gread = None
         cx = _Cx()
 
         def buf_of(v: V):
@@ -1034,6 +1109,8 @@ class _Lowering:
                 if lut not in kern.consts:
                     kern.consts[lut] = f'c{len(kern.consts)}'
                 return kern.consts[lut]
+            if not OPTIONS['dict_lds']:
+                return lut
             if lut not in kern.luts:
                 kern.luts[lut] = (f'L{len(kern.luts)}', enc.k)
             return kern.luts[lut][0]
@@ -1086,9 +1163,18 @@ class _Lowering:
             elif y.kind == 'scalar':
                 s = emit_row(y)
             elif y.kind == 'leaf' and y.obj is not None and id(y.obj) in self.enc:
-                s = var(stmts, leaf_at(y.obj, gi, stmts))
+                if isinstance(gi, _LocalIdx):
+                    s = var(stmts, cx.gread(('enc', y.id), lambda p, t=y.obj:
+                                            leaf_at(t, p, cx.rows), gi, stmts))
+                else:
+                    s = var(stmts, leaf_at(y.obj, gi, stmts))
             elif y.kind == 'leaf' or y.mat:
-                s = var(stmts, f'{buf_of(y)}[{elem(y, gi)}]')
+                bn = buf_of(y)
+                if isinstance(gi, _LocalIdx):
+                    s = var(stmts, cx.gread(('buf', bn), lambda p, bn=bn, y=y:
+                                            f'{bn}[{elem(y, p)}]', gi, stmts))
+                else:
+                    s = var(stmts, f'{bn}[{elem(y, gi)}]')
             elif y.kind == 'ew':
                 a = [emit_at(x, gi, stmts, memo) for x in y.args]
                 s = var(stmts, ew_expr(y, a), 'bool' if is_pred(y) else None)
@@ -1179,7 +1265,22 @@ class _Lowering:
 
             ellv: Dict[str, str] = {}
             tags = {'short': 's', 'unsigned char': 'u', 'int': 'i', 'double': 'd',
-                    'float': 'f'}
+                    'float': 'f', 'unsigned short': 'w'}
+            # halo staging: key -> (LDS name, read(pos) -> C expression)
+            hstage: Dict[tuple, tuple] = {}
+            hinfo: Dict[int, tuple] = {}    # id(sel) -> (halo param, HS)
+            use_halo = ell and OPTIONS['halo'] and not self.no_halo and kern.block == 256
+
+            def gread(key, read, pos, stmts):
+                """A value read at the gathered position `pos`: from the LDS
+                stage when `pos` is a halo-local index (read(p) is the value's
+                expression at global row p), else read(pos) itself."""
+                if not (isinstance(pos, _LocalIdx)):
+                    return read(pos)
+                if key not in hstage:
+                    hstage[key] = (f'H{gid}_{len(hstage)}', read, pos.sel)
+                return f'{hstage[key][0]}[{pos}]'
+            cx.gread = gread
 
             def ell_vec(t, ctype):
                 """Row record of a padded edge array, loaded once per row
@@ -1223,6 +1324,16 @@ class _Lowering:
                 if id(sel) in gidx:
                     return gidx[id(sel)]
                 hi = self.prog.space_len[src.space] - 1
+                lay = _halo_layout(self.prog, sel, red, S, kern.block, int(OPTIONS['halo_max']),
+                                   self.device) if (use_halo and src.space == kern.space) \
+                    else None
+                if lay is not None:
+                    loc, hal, hs = lay
+                    hinfo[id(sel)] = (self._param(kern, hal, 'int'), hs)
+                    li = var(body, f'(int){ell_vec(loc, "unsigned short")}[k]', 'int')
+                    s = _LocalIdx(li, id(sel))
+                    gidx[id(sel)] = s
+                    return s
                 if ell:
                     d = _delta16(sel, red) if OPTIONS['idx16'] else None
                     if d is not None:
@@ -1269,9 +1380,14 @@ class _Lowering:
                     elif src.id in self.remat:
                         s = emit_at(src, gather_index(x.obj, src), body, amem)
                     elif src.kind == 'leaf' and id(src.obj) in self.enc:
-                        s = var(body, leaf_at(src.obj, gather_index(x.obj, src), body))
+                        gi = gather_index(x.obj, src)
+                        s = var(body, gread(('enc', src.id), lambda p, t=src.obj:
+                                            leaf_at(t, p, cx.rows), gi, body))
                     else:
-                        s = var(body, f'{buf_of(src)}[{elem(src, gather_index(x.obj, src))}]')
+                        gi = gather_index(x.obj, src)
+                        bn = buf_of(src)
+                        s = var(body, gread(('buf', bn), lambda p, bn=bn, src=src:
+                                            f'{bn}[{elem(src, p)}]', gi, body))
                 elif x.kind == 'ew':
                     a = [emit_edge(y) for y in x.args]
                     s = var(body, ew_expr(x, a), 'bool' if is_pred(x) else None)
@@ -1315,6 +1431,30 @@ class _Lowering:
             if ell:
                 for u, a in zip(members, accs):
                     body.append(f'  {a} = (k < cn{gid}) ? {a} + {inner[members.index(u)]} : {a};')
+                if hstage:
+                    # every thread of the tile stages its row, the first HS
+                    # threads the tile's halo rows; then the edge slots read LDS
+                    B = kern.block
+                    isz = self.p.dtype.itemsize
+                    lds = sum((B + hinfo[sid][1]) * isz for _, _, sid in hstage.values())
+                    if kern.lds_bytes + lds > 64 * 1024:
+                        raise _NoHalo()
+                    kern.lds_bytes += lds
+                    kern.staged = True
+                    hrow = {}
+                    for sid in dict.fromkeys(v[2] for v in hstage.values()):
+                        hp, hs = hinfo[sid]
+                        hrow[sid] = f'hr{gid}_{len(hrow)}'
+                        cx.rows.append(f'  const int {hrow[sid]} = {hp}[(rb / {B}) * {hs} + '
+                                       f'(threadIdx.x < {hs} ? threadIdx.x : 0)];')
+                    for nm, read, sid in hstage.values():
+                        hs = hinfo[sid][1]
+                        cx.rows.append(f'  __shared__ {ct} {nm}[{B + hs}];')
+                        own = read(cx.R)
+                        cx.rows.append(f'  {nm}[threadIdx.x] = {own};')
+                        far = read(hrow[sid])
+                        cx.rows.append(f'  if (threadIdx.x < {hs}) {nm}[{B} + threadIdx.x] = {far};')
+                    cx.rows.append('  __syncthreads();')
                 cx.rows.append(f'  {{ const int cn{gid} = (int){cntp}[{cx.R}];')
                 cx.rows.append(f'  #pragma unroll')
                 cx.rows.append(f'  for (int k = 0; k < {kmax}; ++k) {{')
@@ -1440,6 +1580,16 @@ class _Lowering:
                     '  const bool live = rb + threadIdx.x < a.n;\n'
                     '  const i64 r = live ? rb + threadIdx.x : a.n - 1;\n')
             tail = '  if (!live) return;\n' + '\n'.join(st)
+        elif OPTIONS['persist'] and kern.luts:
+            # at most `persist` workgroups loop over the row tiles: the
+            # dictionary tables are staged once per workgroup
+            kern.persist = int(OPTIONS['persist'])
+            nt_ = f'((a.n + {kern.block} - 1) / {kern.block})'
+            tl = tile.replace('blockIdx.x', 't_').replace('gridDim.x', nt_)
+            head = (f'  for (i64 t_ = blockIdx.x; t_ < {nt_}; t_ += gridDim.x) {{\n'
+                    f'  const i64 r = {tl} * {kern.block} + threadIdx.x;\n'
+                    '  if (r >= a.n) continue;\n')
+            tail = '\n'.join(st) + '\n  }'
         else:
             head = (f'  const i64 r = {tile} * {kern.block} + threadIdx.x;\n'
                     '  if (r >= a.n) return;\n')
@@ -1551,6 +1701,8 @@ def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
         raw = low.pack(k)
         buf = (ctypes.c_char * len(raw)).from_buffer_copy(raw)
         grid = max(1, (k.n + k.block * k.rpt - 1) // (k.block * k.rpt))
+        if k.persist:
+            grid = min(grid, k.persist)
         launches.append((fn, grid, buf, k.block))
     keep = [v.buf for v in p.values if v.buf is not None]
     eng = FusedEngine(m, launches, keep)

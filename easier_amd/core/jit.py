@@ -71,8 +71,10 @@ def compile(modules: List[_M.Module], backend: str = 'hip', *,
 
     - partition_mode: accepted for compatibility; one GPU holds one partition.
     - reorder: 'rcm' (default) renumbers vertex/cell spaces by reverse
-      Cuthill-McKee; 'none' keeps the user's order. Results are independent
-      of it (Tensor.collect() returns the user's order).
+      Cuthill-McKee; 'blocks' into compact runs of 256 (one workgroup tile
+      of the row kernels; with freeze=True their gathers then read per-tile
+      halos staged in LDS); 'none' keeps the user's order. Results are
+      independent of it (Tensor.collect() returns the user's order).
     - fuse: lower supported forwards into fused kernels (hipRTC); otherwise
       every node runs as its own primitive kernel.
     - load_dir: a directory written by esr.dump(); its renumbering is reused

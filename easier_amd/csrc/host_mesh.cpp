@@ -361,6 +361,48 @@ ESR_HAPI int esr_rcm(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
   ESR_HGUARD_END
 }
 
+ESR_HAPI int esr_block_order(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
+                             int64_t block, int64_t *order) {
+  ESR_HGUARD_BEGIN
+  ESR_HREQUIRE(block >= 1, "esr_block_order: block size %lld < 1", (long long)block);
+  // the RCM sequence decides where blocks start and in which order they come
+  std::vector<int64_t> rcm(n);
+  const int rc = esr_rcm(n, a, b, m, n ? rcm.data() : order);
+  if (rc != ESR_OK) return rc;
+  const Graph g = build_sym(n, a, b, m);
+  std::vector<char> taken(n, 0);
+  int64_t out = 0;
+  std::vector<int64_t> queue;
+  queue.reserve(block);
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t s = rcm[k];
+    if (taken[s]) continue;
+    // a breadth-first ball of untaken vertices around s, in BFS order
+    // (neighbours in adjacency order), as large as the rest of the current
+    // run of `block` vertices: balls that end early are followed by balls
+    // that fill the run, so every run starts a ball
+    const int64_t limit = block - out % block;
+    queue.clear();
+    queue.push_back(s);
+    taken[s] = 1;
+    for (size_t h = 0; h < queue.size() && (int64_t)queue.size() < limit; ++h) {
+      const int64_t v = queue[h];
+      for (int64_t j = g.ptr[v]; j < g.ptr[v + 1] && (int64_t)queue.size() < limit; ++j) {
+        const int64_t w = g.adj[j];
+        if (!taken[w]) {
+          taken[w] = 1;
+          queue.push_back(w);
+        }
+      }
+    }
+    for (int64_t v : queue) order[out++] = v;
+  }
+  ESR_HREQUIRE(out == n, "esr_block_order: internal count %lld != %lld", (long long)out,
+               (long long)n);
+  return ESR_OK;
+  ESR_HGUARD_END
+}
+
 ESR_HAPI int esr_bandwidth(const int64_t *a, const int64_t *b, int64_t m, int64_t *bw) {
   ESR_HREQUIRE(bw != nullptr && (m == 0 || (a && b)), "esr_bandwidth: null pointer");
   int64_t w = 0;

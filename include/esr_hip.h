@@ -229,6 +229,16 @@ int esr_csr_from_idx(const int64_t *idx, int64_t m, int64_t n, int64_t *rowptr,
  * order[new] = old.                                                           */
 int esr_rcm(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
             int64_t *order);
+/* Blocks for locality within one partition (the reference partitions
+ * vertices among ranks, core/distpart.py:661-775, and keeps each rank's
+ * order; on one GPU the unit of locality is a workgroup's tile of rows):
+ * order[new] = old, consecutive runs of `block` vertices are breadth-first
+ * balls grown over not yet placed vertices, started at the first unplaced
+ * vertex of the RCM order (so blocks follow the RCM sweep), each ball in BFS
+ * order. A ball that runs out of reachable vertices ends early; the balls
+ * after it fill the rest of its run, so every run of `block` starts a ball. */
+int esr_block_order(int64_t n, const int64_t *a, const int64_t *b, int64_t m,
+                    int64_t block, int64_t *order);
 /* Largest |row(e) - col(e)| over the pairs: the bandwidth RCM minimises.     */
 int esr_bandwidth(const int64_t *a, const int64_t *b, int64_t m,
                   int64_t *bw);

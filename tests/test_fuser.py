@@ -294,3 +294,33 @@ def test_frozen_encoding_skips_written_and_many_valued_tensors():
     assert {id(eqn.h), id(eqn.uh), id(eqn.vh)} <= written
     assert id(eqn.sx) not in written
     _rtc_ok(src)
+
+
+def test_halo_layout_indexes_every_source_row():
+    """reorder='blocks' + freeze: every edge slot's LDS index points at its
+    source row -- the tile's own rows first, then the tile's halo list."""
+    mesh = wmesh.tri2(50)
+    geo = oswe.geometry(mesh)
+    eqn = wswe.ShallowWaterEquation(mesh, geo, dt=0.001, g=9.81, dtype=torch.float64)
+    prog = compile_program([eqn], 'blocks', torch.device('cpu'))
+    sel, red = eqn.gather_src, eqn.scatter
+    B, S = 256, 4
+    lay = fuser._halo_layout(prog, sel, red, S, B, 256, torch.device('cpu'))
+    assert lay is not None
+    loc, halo, hs = (x.numpy() if hasattr(x, 'numpy') else x for x in lay)
+    slot = fuser._ell_slots(prog, red, S)
+    n = slot.shape[0]
+    idx = np.asarray(sel.easier_host_idx)
+    loc = loc.reshape(n, S).astype(np.int64)
+    halo = halo.reshape(-1, hs)
+    for r in range(n):
+        t = r // B
+        for k in range(S):
+            e = slot[r, k]
+            if e < 0:
+                continue
+            i = loc[r, k]
+            got = t * B + i if i < B else halo[t, i - B]
+            assert got == idx[e], (r, k)
+    inside = np.mean(loc[slot >= 0] < B)
+    assert inside > 0.7

@@ -187,20 +187,24 @@ def test_swe_steps_match_oracle(gpu, dtype, gen, g, fuse):
             assert torch.equal(got, want), k
 
 
+@pytest.mark.parametrize('reorder', ['rcm', 'blocks'])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 @pytest.mark.parametrize('gen', ['tri2', 'tri4', 'tri2_jitter'])
-def test_frozen_swe_steps_match_oracle(gpu, dtype, gen):
+def test_frozen_swe_steps_match_oracle(gpu, dtype, gen, reorder):
     """compile(..., freeze=True): the geometry is read through dictionaries
-    (uniform meshes) or as arrays (jittered: too many distinct values); 30
-    steps bit-identical to the oracle in float64, within 1e-5 in float32."""
-    print(f'frozen swe {gen} {dtype}', flush=True)
+    (uniform meshes) or as arrays (jittered: too many distinct values), and
+    with reorder='blocks' the gathers read per-tile LDS halos; 30 steps
+    bit-identical to the oracle in float64, within 1e-5 in float32."""
+    print(f'frozen swe {gen} {dtype} reorder={reorder}', flush=True)
     mesh = wmesh.tri2(64, jitter=0.2, seed=3) if gen == 'tri2_jitter' else \
         getattr(wmesh, gen)(64)
     geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
     dt = wswe.cfl_dt(mesh, geo['h'], 9.81)
     eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=9.81, dtype=dtype)
-    esr.compile([eqn], 'hip', freeze=True)
+    esr.compile([eqn], 'hip', freeze=True, reorder=reorder)
     luts = sum(len(k.luts) + len(k.consts) for k in eqn.easier_engine.kernels)
+    if reorder == 'blocks':               # the gathers go through LDS halos
+        assert 'H1_0' in eqn.easier_engine.source
     assert (luts > 0) == (gen != 'tri2_jitter') or dtype == torch.float32
     ref = oswe.SWE(mesh, geo, dt=dt, g=9.81, dtype=dtype)
     for _ in range(30):

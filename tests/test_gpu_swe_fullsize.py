@@ -55,14 +55,15 @@ def test_full_size_step_bit_identical_and_mass_conserved(gpu, full):
     assert abs(m20 - m0) <= 1e-12 * abs(m0), (m0, m20)
 
 
-def test_full_size_frozen_step_bit_identical(gpu, full):
+@pytest.mark.parametrize('reorder', ['rcm', 'blocks'])
+def test_full_size_frozen_step_bit_identical(gpu, full, reorder):
     """compile(..., freeze=True) reads the geometry through dictionaries
     (the uniform mesh's alpha, sx, sy take few values; area one): the same
     bits as the oracle after two steps."""
     mesh, geo, dt = full
-    print(f'swe full size {N}x{N} frozen steps vs oracle', flush=True)
+    print(f'swe full size {N}x{N} frozen steps vs oracle, reorder={reorder}', flush=True)
     eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=G, dtype=torch.float64)
-    esr.compile([eqn], 'hip', freeze=True)
+    esr.compile([eqn], 'hip', freeze=True, reorder=reorder)
     assert any(k.luts for k in eqn.easier_engine.kernels)
     ref = oswe.SWE(mesh, geo, dt=dt, g=G, dtype=torch.float64)
     for _ in range(2):

@@ -94,3 +94,44 @@ def test_rcm_is_a_bandwidth_reducing_permutation():
     bw_scipy = compiler.bandwidth(sinv[src], sinv[dst])
     assert bw <= bw_scipy + 2
     assert bw < compiler.bandwidth(src, dst) / 5
+
+
+@pytest.mark.parametrize('mesh_kind,n,B', [('tri2', 40, 256), ('tri4', 23, 64), ('tri2', 7, 1000)])
+def test_block_order_is_a_permutation_of_compact_blocks(mesh_kind, n, B):
+    """esr_block_order: a permutation whose consecutive runs of B vertices
+    are connected (each run is a breadth-first ball); on a mesh most edges
+    stay inside a run, far more than in the RCM order."""
+    m = getattr(wmesh, mesh_kind)(n)
+    nc = m['cells'].shape[0]
+    o = compiler.block_order(nc, m['src'], m['dst'], B)
+    assert sorted(o.tolist()) == list(range(nc))
+    pos = np.empty(nc, np.int64)
+    pos[o] = np.arange(nc)
+    ps, pd = pos[m['src']], pos[m['dst']]
+    inside = np.mean(ps // B == pd // B)
+    r = compiler.rcm_order(nc, m['src'], m['dst'])
+    posr = np.empty(nc, np.int64)
+    posr[r] = np.arange(nc)
+    inside_rcm = np.mean(posr[m['src']] // B == posr[m['dst']] // B)
+    if nc > B:
+        assert inside > 0.7 and inside >= inside_rcm, (inside, inside_rcm)
+    # runs are compact: mostly one connected piece (a run holds more only
+    # where a ball ran out of unplaced vertices)
+    adj = {}
+    for a, b in zip(m['src'].tolist(), m['dst'].tolist()):
+        adj.setdefault(a, []).append(b)
+    pieces = []
+    for k in range(0, nc, B):
+        blk = set(o[k:k + B].tolist())
+        left, count = set(blk), 0
+        while left:
+            count += 1
+            stack = [left.pop()]
+            while stack:
+                u = stack.pop()
+                for w in adj.get(u, []):
+                    if w in left:
+                        left.discard(w)
+                        stack.append(w)
+        pieces.append(count)
+    assert np.mean(pieces) <= 2.0, pieces
