@@ -39,6 +39,7 @@ from torch import fx
 
 from .. import _native as nat
 from . import module as _M
+from . import compiler as _compiler
 from .compiler import CompiledProgram, resolve_attr
 from .utils import logger
 
@@ -143,7 +144,7 @@ OPTIONS = {'stage_edges': False, 'nt_loads': False, 'nt_stores': True, 'block': 
            'xcd_remap': False, 'sparse_rows': True, 'idx16': True, 'xcd_group': 16,
            'rows_per_thread': 1, 'pack_intermediates': False, 'waves_per_eu': 0,
            'remat': True, 'late_eval': True, 'dict_max': 256, 'dict_min_rows': 4096,
-           'ell': True, 'dict_lds': True, 'persist': 0, 'halo': True, 'halo_max': 256}
+           'ell': True, 'dict_lds': True, 'persist': 0, 'halo': True, 'halo_max': 1024}
 _LDS_BUDGET = 48 * 1024
 
 
@@ -1269,7 +1270,8 @@ class _Lowering:
             # halo staging: key -> (LDS name, read(pos) -> C expression)
             hstage: Dict[tuple, tuple] = {}
             hinfo: Dict[int, tuple] = {}    # id(sel) -> (halo param, HS)
-            use_halo = ell and OPTIONS['halo'] and not self.no_halo and kern.block == 256
+            use_halo = ell and OPTIONS['halo'] and not self.no_halo and \
+                kern.block == _compiler.BLOCK_ROWS
 
             def gread(key, read, pos, stmts):
                 """A value read at the gathered position `pos`: from the LDS
@@ -1324,7 +1326,8 @@ class _Lowering:
                 if id(sel) in gidx:
                     return gidx[id(sel)]
                 hi = self.prog.space_len[src.space] - 1
-                lay = _halo_layout(self.prog, sel, red, S, kern.block, int(OPTIONS['halo_max']),
+                lay = _halo_layout(self.prog, sel, red, S, kern.block,
+                                   min(int(OPTIONS['halo_max']), kern.block),
                                    self.device) if (use_halo and src.space == kern.space) \
                     else None
                 if lay is not None:
