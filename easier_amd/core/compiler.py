@@ -57,6 +57,7 @@ class EasierProxy(Proxy):
 # Captured at import: while tracing, the autowrapped module's names are
 # replaced by FX wrappers, so the classes must not be looked up there.
 _LEAF_TYPES = (_M.Module, _M.Selector, _M.Reducer)
+_TENSOR_TYPE = _M.Tensor
 
 
 class EasierTracer(Tracer):
@@ -74,6 +75,16 @@ class EasierTracer(Tracer):
 
     def proxy(self, node):
         return EasierProxy(node, self)
+
+    def create_arg(self, a):
+        # An esr.Tensor a forward reaches through a closure rather than an
+        # attribute (M = lambda r: r / A.Av passed to a solver whose module
+        # does not own A) is registered on the traced module, so the graph
+        # names it by an attribute path like every other tensor.
+        if isinstance(a, _TENSOR_TYPE) and \
+                all(p is not a for _, p in self.root.named_parameters(remove_duplicate=False)):
+            self.root.register_parameter(f'easier_captured_{len(self.root._parameters)}', a)
+        return super().create_arg(a)
 
 
 def get_easier_objects(top_modules) -> 'OrderedDict':

@@ -1,9 +1,10 @@
 """Fused device iterations for CG and GMRES on a compiled Linsys.
 
-After esr.compile(..., 'hip'), CG/GMRES check whether their operator is a
-plain Linsys (reducer(selector(x) * Ae) + x * Av) whose Reducer fixes its edge
-order (so the edge list is a row-sorted CSR) and whose preconditioner is the
-identity or preconditioner.Jacobi. If so, solve() runs:
+After esr.compile(..., 'hip'), CG/GMRES check whether their traced forwards
+apply an operator of the Linsys form (reducer(selector(x) * Ae) + x * Av,
+recognised in the graph: solver/recognize.py) whose Reducer fixes its edge
+order (so the edge list is a row-sorted CSR) and a preconditioner that
+computes M(r) = r or M(r) = r / d. If so, solve() runs:
 
 CG: one init kernel, then chunks of iterations, each iteration three kernels
     (SpMV + p.Ap; x/r/z update + r.r, r.z; loop test + p update) with every
@@ -46,40 +47,57 @@ nat.register_signatures({
 })
 
 
+def _parts(A):
+    """{'selector', 'reducer', 'Ae', 'Av'} of a Linsys, or the dict itself
+    (an operator recognised in a traced graph, solver/recognize.py)."""
+    if isinstance(A, dict):
+        return A
+    if type(A) is Linsys:
+        return dict(selector=A.selector, reducer=A.reducer, Ae=A.Ae, Av=A.Av)
+    return None
+
+
 class LinsysCSR:
-    """The device CSR view of a compiled Linsys: rowptr, col (the Selector's
+    """The device CSR view of a compiled Linsys (or of an operator of the
+    same form recognised in the traced forward): rowptr, col (the Selector's
     renumbered index in row-sorted edge order), Ae, Av."""
 
-    def __init__(self, A: Linsys):
-        self.rowptr = A.reducer.easier_rowptr
-        self.col = A.selector.idx
-        self.Ae = A.Ae.data
-        self.Av = A.Av.data
-        self.n = A.reducer.n
+    def __init__(self, A):
+        P = _parts(A)
+        self.rowptr = P['reducer'].easier_rowptr
+        self.col = P['selector'].idx
+        self.Ae = P['Ae'].data
+        self.Av = P['Av'].data
+        self.n = P['reducer'].n
         self.dtype = self.Av.dtype
 
     @staticmethod
     def applicable(A, x: esr.Tensor, prog) -> Optional[str]:
         """None if the fused path applies, else the reason it does not."""
-        if type(A) is not Linsys:
-            return 'operator is not a Linsys'
-        sel, red = A.selector, A.reducer
+        P = _parts(A)
+        if P is None:
+            return 'operator is not reducer(selector(x) * Ae) + x * Av'
+        sel, red = P['selector'], P['reducer']
         if not isinstance(sel, esr.Selector) or not isinstance(red, esr.Reducer):
             return 'Linsys without an esr Selector/Reducer'
         if red.reduce != 'sum':
             return 'Reducer is not a sum'
         if red.easier_perm is not None:
             return 'the Reducer does not fix the edge order'
-        for t in (A.Av, A.Ae, x):
+        Av, Ae = P['Av'], P['Ae']
+        for t in (Av, Ae, x):
             if not isinstance(t, esr.Tensor) or not t.is_partition:
                 return 'Av/Ae/x are not partition tensors'
-        if A.Av.data.dim() != 1 or A.Ae.data.dim() != 1 or x.data.dim() != 1:
+        if Av.data.dim() != 1 or Ae.data.dim() != 1 or x.data.dim() != 1:
             return 'only vectors (one value per vertex/edge) are fused'
-        dts = {A.Av.dtype, A.Ae.dtype, x.dtype}
-        if len(dts) != 1 or A.Av.dtype not in (torch.float32, torch.float64):
+        dts = {Av.dtype, Ae.dtype, x.dtype}
+        if len(dts) != 1 or Av.dtype not in (torch.float32, torch.float64):
             return 'mixed or non-float dtypes'
-        if A.Ae.data.shape[0] != sel.idx.shape[0] or red.n != x.data.shape[0]:
+        if Ae.data.shape[0] != sel.idx.shape[0] or red.n != x.data.shape[0]:
             return 'shape mismatch'
+        sa = prog.spaces
+        if sa.root(sa.tensor_var[Av]) != sa.root(sa.tensor_var[x]):
+            return 'Av and x are different spaces'
         sa = prog.spaces
         sv = sa.root(sa.sel_vars[sel][0])
         rv = sa.root(sa.red_vars[red][1])
@@ -87,6 +105,14 @@ class LinsysCSR:
         if not (sv == rv == xv):
             return 'selector source, reducer target and x are different spaces'
         return None
+
+
+def _diag_ok(d, x: esr.Tensor, prog) -> bool:
+    """A Jacobi diagonal the fused kernels take: a partition vector in x's
+    space and dtype."""
+    sa = prog.spaces
+    return isinstance(d, esr.Tensor) and d.is_partition and d.data.dim() == 1 and \
+        d.dtype == x.dtype and sa.root(sa.tensor_var[d]) == sa.root(sa.tensor_var[x])
 
 
 def _precond_diag(M, x: esr.Tensor, prog):
@@ -136,10 +162,10 @@ class _Graph:
 class FusedCG:
     CHUNK = 32
 
-    def __init__(self, cg, prog):
+    def __init__(self, cg, prog, op=None, diag=None):
         self.cg = cg
-        self.op = LinsysCSR(cg.A)
-        self.diag = _precond_diag(cg.M, cg.x, prog)[1]
+        self.op = LinsysCSR(op if op is not None else cg.A)
+        self.diag = diag.data if diag is not None else None
         dev = cg.x.device
         n = self.op.n
         sb, wb = ctypes.c_int64(), ctypes.c_int64()
@@ -207,23 +233,31 @@ class FusedCG:
 
 
 def make_fused_cg(cg, prog):
-    why = LinsysCSR.applicable(cg.A, cg.x, prog)
-    ok, _ = _precond_diag(cg.M, cg.x, prog)
-    if why is None and not ok:
-        why = 'preconditioner is neither identity nor Jacobi'
+    """The fused iteration when Step applies reducer(selector(p) * Ae) +
+    p * Av and Update computes z = r or z = r / d (recognised in the traced
+    graphs, whatever objects A and M are), else None."""
+    from .recognize import cg_preconditioner, graph_of, match_linsys
+    g_step, g_upd = graph_of(prog, cg.step), graph_of(prog, cg.update)
+    op = match_linsys(cg.step, g_step, cg.p) if g_step is not None else None
+    why = LinsysCSR.applicable(op, cg.x, prog)
+    pre = cg_preconditioner(cg.update, g_upd, cg.r, cg.z) if g_upd is not None else None
+    if why is None and pre is None:
+        why = 'preconditioner is neither z = r nor z = r / d'
+    if why is None and pre[0] == 'jacobi' and not _diag_ok(pre[1], cg.x, prog):
+        why = 'the Jacobi diagonal is not a vector of x\'s space and dtype'
     if why is None and (cg.b.dtype != cg.x.dtype or cg.b.data.shape != cg.x.data.shape):
         why = 'b and x differ in dtype or shape'
     if why is not None:
         logger.info(f'CG: generic compiled path ({why})')
         return None
-    return FusedCG(cg, prog)
+    return FusedCG(cg, prog, op, pre[1])
 
 
 class FusedGMRES:
-    def __init__(self, gm, prog):
+    def __init__(self, gm, prog, op=None, diag=None):
         self.gm = gm
-        self.op = LinsysCSR(gm.A)
-        self.diag = _precond_diag(gm.M, gm.x, prog)[1]
+        self.op = LinsysCSR(op if op is not None else gm.A)
+        self.diag = diag.data if diag is not None else None
         dev = gm.x.device
         n, m = self.op.n, gm.restart
         sb, wb = ctypes.c_int64(), ctypes.c_int64()
@@ -303,13 +337,23 @@ class FusedGMRES:
 
 
 def make_fused_gmres(gm, prog):
-    why = LinsysCSR.applicable(gm.A, gm.x, prog)
-    ok, _ = _precond_diag(gm.M, gm.x, prog)
-    if why is None and not ok:
-        why = 'preconditioner is neither identity nor Jacobi'
+    """The fused Arnoldi cycles when UpdateRnorm applies
+    reducer(selector(x) * Ae) + x * Av and UpdateX adds M(dx) = dx or
+    dx / d (recognised in the traced graphs), else None."""
+    from .recognize import gmres_preconditioner, graph_of, match_linsys
+    g_res = graph_of(prog, gm.update_rnorm)
+    op = match_linsys(gm.update_rnorm, g_res, gm.x) if g_res is not None else None
+    why = LinsysCSR.applicable(op, gm.x, prog)
+    ux = gm.update_x[0]
+    g_ux = graph_of(prog, ux)
+    pre = gmres_preconditioner(ux, g_ux, gm.x) if g_ux is not None else None
+    if why is None and pre is None:
+        why = 'preconditioner is neither M(v) = v nor M(v) = v / d'
+    if why is None and pre[0] == 'jacobi' and not _diag_ok(pre[1], gm.x, prog):
+        why = 'the Jacobi diagonal is not a vector of x\'s space and dtype'
     if why is None and gm.restart > 32:
         why = 'restart > 32'
     if why is not None:
         logger.info(f'GMRES: generic compiled path ({why})')
         return None
-    return FusedGMRES(gm, prog)
+    return FusedGMRES(gm, prog, op, pre[1])

@@ -134,3 +134,55 @@ def test_fv_poisson_integration_solve(gpu, fv100, solver, tol):
     assert abs(info['iters'] - info_o['iters']) <= 1
     A = _sparse(ref['Ac'], ref['Af'], mesh['src'], mesh['dst'])
     _check_solution(A, b_o, x, x_o, f'{solver} {tol}')
+
+
+class _UserJacobi(torch.nn.Module):
+    def __init__(self, d):
+        super().__init__()
+        self.d = d
+
+    def forward(self, r):
+        return r / self.d
+
+
+class _UserOperator(torch.nn.Module):
+    """The Linsys product written by a user (operands in another order)."""
+
+    def __init__(self, Av, Ae, sel, red):
+        super().__init__()
+        self.Av, self.Ae, self.sel, self.red = Av, Ae, sel, red
+
+    def forward(self, x):
+        return self.Av * x + self.red(self.Ae * self.sel(x))
+
+
+@pytest.mark.parametrize('solver', ['cg', 'gmres'])
+@pytest.mark.parametrize('kind', ['lambda', 'module', 'user_operator'])
+def test_reference_style_callables_take_the_fused_path(gpu, kind, solver):
+    """M = lambda r: r / A.Av, a user nn.Module M, or a user-written operator
+    (the reference's solvers take any, cg.py:86, gmres.py:152): recognised
+    in the traced graphs, solved by the fused device iteration, iterations
+    and solution as the oracle's."""
+    print(f'drop-in {kind} {solver}', flush=True)
+    mesh, sys_ = wp.config1_system(40, seed=2)
+    eqn = wp.PoissonP1(sys_)
+    A = eqn.A if kind != 'user_operator' else _UserOperator(eqn.Av, eqn.Ae, eqn.selector,
+                                                             eqn.reducer)
+    M = _UserJacobi(eqn.Av) if kind == 'module' else (lambda r: r / eqn.Av)
+    sol = CG(A, eqn.b, eqn.x, M) if solver == 'cg' else GMRES(A, eqn.b, eqn.x, M, restart=20)
+    esr.compile([eqn, sol], 'hip')
+    assert type(sol.easier_fused).__name__ == ('FusedCG' if solver == 'cg' else 'FusedGMRES')
+    info = sol.solve(rtol=1e-9, maxiter=3000)
+    n = sys_['Av'].shape[0]
+    row, col = torch.from_numpy(sys_['row']), torch.from_numpy(sys_['col'])
+    Av = torch.from_numpy(sys_['Av'])
+    A_o = osol.Linsys(Av, torch.from_numpy(sys_['Ae']), col, row, n)
+    b_o = torch.from_numpy(sys_['b'])
+    run = osol.cg if solver == 'cg' else osol.gmres
+    info_o, x_o = run(A_o, b_o, torch.zeros(n, dtype=torch.float64), M=lambda r: r / Av,
+                      rtol=1e-9, maxiter=3000)
+    print(f'  iters {info["iters"]} (oracle {info_o["iters"]})', flush=True)
+    assert abs(info['iters'] - info_o['iters']) <= 1
+    assert info['residual'] <= 1e-9 * float(b_o.norm())
+    _check_solution(_sparse(sys_['Av'], sys_['Ae'], sys_['row'], sys_['col']), sys_['b'],
+                    eqn.x.collect(), x_o, f'{kind} {solver}')
