@@ -116,4 +116,41 @@ __device__ __forceinline__ T linsys_row(int64_t r, const int32_t *__restrict__ r
   return s + src(r) * Av[r];
 }
 
+// The same row with the slot count chosen per wavefront: K = 4, 6 or 8
+// independent slots, the smallest that covers the longest row of the
+// lanes executing it (wave-uniform, from ballots), so rows of a mesh whose
+// vertices have 6 neighbours do not issue the loads of two empty slots. The
+// fold is the same left-to-right sum of the row's entries.
+template <int K, typename T, typename Src>
+__device__ __forceinline__ T linsys_slots(int32_t b, int32_t e, const int32_t *__restrict__ col,
+                                          const T *__restrict__ Ae, Src src) {
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int32_t j = b + k;
+    const int32_t jc = j < e ? j : (e > b ? e - 1 : b);
+    const T v = src(col[jc]) * Ae[jc];
+    s = j < e ? s + v : s;
+  }
+  for (int32_t j = b + K; j < e; ++j) s = s + src(col[j]) * Ae[j];
+  return s;
+}
+
+template <typename T, typename Src>
+__device__ __forceinline__ T linsys_row_wave(int64_t r, const int32_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ col,
+                                             const T *__restrict__ Ae, const T *__restrict__ Av,
+                                             Src src) {
+  const int32_t b = rowptr[r], e = rowptr[r + 1];
+  T s = T(0);
+  if (__ballot(e - b > 6)) {
+    if (e > b) s = linsys_slots<8, T>(b, e, col, Ae, src);
+  } else if (__ballot(e - b > 4)) {
+    if (e > b) s = linsys_slots<6, T>(b, e, col, Ae, src);
+  } else {
+    if (e > b) s = linsys_slots<4, T>(b, e, col, Ae, src);
+  }
+  return s + src(r) * Av[r];
+}
+
 } // namespace esr

@@ -101,9 +101,10 @@ __device__ __forceinline__ PcgTest pcg_test(const double *__restrict__ part, int
 // One PCG iteration is two kernels on small systems, three on large ones
 // (kFuseUpdateMax); both forms run the same operations on the same values.
 //
-// At 4.2 M vertices the three-kernel iteration is 170 us: kernel 1 103 us
+// At 4.2 M vertices the three-kernel iteration was 170 us: kernel 1 103 us
 // (SpMV, 4.1 TB/s of its 420 MB), kernel 2 48 us (5.6 TB/s), the Update
-// 19 us (5.2 TB/s). Measured alternatives for the SpMV, all slower or equal:
+// 19 us (5.2 TB/s); with the wave-uniform slot count of the product it is
+// 158 us. Measured alternatives for the SpMV, all slower or equal:
 // groups of 16 or 64 row tiles per XCD (1.539-1.543 s per solve, unchanged),
 // a contiguous chunk of rows per workgroup (1.62 s), each wavefront's CSR
 // segment staged in LDS before the gathers (1.56 s; float32 1.49 vs 1.31 s).
@@ -116,7 +117,7 @@ __device__ __forceinline__ PcgTest pcg_test(const double *__restrict__ part, int
 // read from one direction buffer and written to the other, so no workgroup
 // overwrites a value another one still gathers. After Init, a re-arm or the
 // separate Update kernel (fresh) the step starts from p as it is.
-template <typename T>
+template <typename T, bool FUSED>
 __global__ __launch_bounds__(kRB) void k_pcg_step1(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ z,
@@ -128,18 +129,18 @@ __global__ __launch_bounds__(kRB) void k_pcg_step1(
   const T *__restrict__ pin = cur ? p1 : p0;
   T *__restrict__ pout = cur ? p0 : p1;
   double acc[1] = {0.0};
-  if (st->fresh) {
+  if (!FUSED || st->fresh) {
     if (lead) {
       st->k_next = st->k_cur;
       st->rz_next = st->rz_cur;
       st->pbuf_next = cur;
     }
     ESR_GRID_ROWS(i, n) {
-      const T y = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return pin[c]; });
+      const T y = linsys_row_wave<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return pin[c]; });
       Ap[i] = y;
       acc[0] += (double)(pin[i] * y);
     }
-  } else {
+  } else if constexpr (FUSED) {
     const PcgTest t = pcg_test(part_in, gridDim.x, st->k_cur, st, lead);
     if (t.exit) return;
     const T beta = (T)(t.rz / st->rz_cur);
@@ -174,10 +175,10 @@ __global__ __launch_bounds__(kRB) void k_pcg_step2(int64_t n, const T *__restric
                                                   const T *__restrict__ p1,
                                                   const T *__restrict__ Ap, PcgState *st,
                                                   const double *__restrict__ part_in,
-                                                  double *__restrict__ part_out) {
+                                                  int64_t g_in, double *__restrict__ part_out) {
   if (st->done) return;
   double pAp[1];
-  sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, pAp);
+  sum_partials<1>(part_in, g_in, 0, g_in, pAp);
   const double rz = st->rz_next;
   const double alpha_d = rz / pAp[0];
   const T *__restrict__ p = st->pbuf_next ? p1 : p0;
@@ -537,8 +538,19 @@ using namespace esr;
 // 4.2 M: 170 -> 180 us).
 constexpr int64_t kFuseUpdateMax = 1 << 17;
 
+// Grid of the product kernel of the three-kernel iteration (its partials
+// are summed by the next kernel in every workgroup, so it may differ from
+// the other kernels' grid). At 4.2 M vertices caps of 1,024 / 4,096 / 8,192
+// workgroups gave 158.0 / 159.4 / 160.2 us per iteration (float64) and
+// 97.4 / 98.6 / 99.5 us (float32): more rows in flight do not help, the
+// product is bound by the number of load instructions per row, which the
+// wave-uniform slot count (linsys_row_wave) cut from 8 to 6 per row on the
+// 2-triangle mesh (167.6 -> 158.0 us, 107.3 -> 97.4 us).
+static unsigned spmv_grid(int64_t n) { return red_grid(n); }
+
 static int64_t pcg_parts_bytes(int64_t n) {
-  return (3 * (int64_t)red_grid(n) * (int64_t)sizeof(double) + 255) / 256 * 256;
+  const int64_t g = red_grid(n), g1 = spmv_grid(n);
+  return ((2 * g + (g1 > g ? g1 : g)) * (int64_t)sizeof(double) + 255) / 256 * 256;
 }
 
 ESR_API int esr_pcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes) {
@@ -584,17 +596,23 @@ ESR_API int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr, const int
               "esr_pcg_steps: null argument");
   hipStream_t s = as_stream(stream);
   const unsigned g = red_grid(n);
-  double *part1 = (double *)ws, *part2 = part1 + g;
-  void *p1 = (char *)ws + pcg_parts_bytes(n);
   const bool fuse = n <= kFuseUpdateMax;
+  const unsigned g1 = fuse ? g : spmv_grid(n);
+  double *part2 = (double *)ws, *part1 = part2 + 2 * (int64_t)g;
+  void *p1 = (char *)ws + pcg_parts_bytes(n);
   for (int64_t it = 0; it < count; ++it) {
     ESR_FLOAT_DISPATCH(dtype, {
-      k_pcg_step1<T><<<g, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
-                                       (const T *)z, (T *)p, (T *)p1, (T *)Ap,
-                                       (PcgState *)state, part2, part1);
+      if (fuse)
+        k_pcg_step1<T, true><<<g1, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
+                                               (const T *)z, (T *)p, (T *)p1, (T *)Ap,
+                                               (PcgState *)state, part2, part1);
+      else
+        k_pcg_step1<T, false><<<g1, kRB, 0, s>>>(n, rowptr, col, (const T *)Ae, (const T *)Av,
+                                                (const T *)z, (T *)p, (T *)p1, (T *)Ap,
+                                                (PcgState *)state, part2, part1);
       k_pcg_step2<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
                                        (const T *)p, (const T *)p1, (const T *)Ap,
-                                       (PcgState *)state, part1, part2);
+                                       (PcgState *)state, part1, (int64_t)g1, part2);
       if (!fuse)
         k_pcg_update<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (T *)p1, (PcgState *)state,
                                           part2);
