@@ -827,16 +827,36 @@ class Kernel:
         return tot
 
 
+class BoundTensors:
+    """The tensors whose device addresses packed kernel arguments or a
+    captured HIP graph hold, with those addresses. check() raises before a
+    launch if one of them has been given new storage (its .data reassigned),
+    which the launch would otherwise never see."""
+
+    def __init__(self, tensors):
+        self.pairs = [(t, t.data_ptr()) for t in tensors if t is not None]
+
+    def check(self):
+        for t, ptr in self.pairs:
+            if t.data_ptr() != ptr:
+                raise RuntimeError(
+                    f'the storage of a compiled tensor {tuple(t.shape)} was replaced '
+                    'after compile() (its .data reassigned); the compiled kernels '
+                    'still address the old storage -- update it in place '
+                    '(t.data.copy_(...)) or compile again')
+
+
 class FusedEngine:
     """Executes the fused kernels of one forward; captured as a HIP graph
     after the first call."""
 
-    def __init__(self, module, launches, keepalive):
+    def __init__(self, module, launches, keepalive, tensors=()):
         self.module = module
         self.launches = launches          # (fn, grid, argbytes, block)
         self.keepalive = keepalive
         self.graph = None
         self.calls = 0
+        self.bound = BoundTensors(tensors)
 
     def _launch_all(self, stream):
         for fn, grid, buf, block in self.launches:
@@ -844,6 +864,7 @@ class FusedEngine:
                      ctypes.addressof(buf), ctypes.sizeof(buf))
 
     def forward(self):
+        self.bound.check()
         self.calls += 1
         if self.graph is not None:
             nat.call('esr_graph_launch', self.graph.exec, nat.stream_ptr())
@@ -898,11 +919,16 @@ class _Lowering:
         itself); computed once per pair, as every candidate plan asks."""
         a, b = getattr(sel, 'easier_host_idx', None), getattr(red, 'easier_host_idx', None)
         cache = sel.__dict__.setdefault('easier_rowlocal', {})
-        key = (id(red), id(a), id(b), red.easier_perm is None)
-        if key not in cache:
-            cache[key] = a is not None and b is not None and a.shape == b.shape and \
+        # the entry keeps the objects it was computed from and is used only
+        # while they are the same objects (ids alone could be reused)
+        hit = cache.get(id(red))
+        if hit is None or hit[0] is not red or hit[1] is not a or hit[2] is not b or \
+                hit[3] != (red.easier_perm is None):
+            same = a is not None and b is not None and a.shape == b.shape and \
                 red.easier_perm is None and np.array_equal(a, b)
-        return cache[key]
+            hit = (red, a, b, red.easier_perm is None, same)
+            cache[id(red)] = hit
+        return hit[4]
 
     def _remat_set(self) -> set:
         """Ids of gathered elementwise values that their gathers recompute
@@ -1708,7 +1734,9 @@ def fuse_module(m: _M.Module, g: fx.Graph, prog: CompiledProgram):
             grid = min(grid, k.persist)
         launches.append((fn, grid, buf, k.block))
     keep = [v.buf for v in p.values if v.buf is not None]
-    eng = FusedEngine(m, launches, keep)
+    tensors = list({id(v.obj): v.obj for v in p.values
+                    if v.kind in ('leaf', 'scalar', 'store')}.values())
+    eng = FusedEngine(m, launches, keep, tensors)
     eng.source = src
     eng.kernels = low.kernels
     return eng

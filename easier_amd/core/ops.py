@@ -126,6 +126,9 @@ def ewise(op: str, *operands, out: Optional[torch.Tensor] = None):
         _launch('copy', [src], out, in_dtype)
         return out
     cdt = _compute_dtype(op, operands)
+    if op == 'pow' and not cdt.is_floating_point and _is_scalar(operands[1]) and \
+            operands[1] < 0:
+        raise RuntimeError('Integers to negative integer powers are not allowed.')
     shape = torch.broadcast_shapes(*[tuple(t.shape) for t in tensors]) \
         if tensors else tuple(out.shape)
     inputs = []

@@ -546,9 +546,22 @@ template <typename T> __device__ __forceinline__ T ew_pow(T a, T b) {
     if (b == T(-2)) return T(1) / (a * a);
     return pow(a, b);
   } else {
-    T r = 1;
-    for (T k = 0; k < b; ++k) r *= a;
-    return r;
+    // torch's integer power: a negative exponent gives 1 for a = 1, +-1 for
+    // a = -1 and 0 otherwise; else exponentiation by squaring (at most 64
+    // steps), wrapping modulo 2^bits like torch's integer arithmetic
+    if (b < T(0)) {
+      if (a == T(1)) return T(1);
+      if (a == T(-1)) return (b & T(1)) ? T(-1) : T(1);
+      return T(0);
+    }
+    using U = typename std::make_unsigned<T>::type;
+    U base = (U)a, r = 1, e = (U)b;
+    while (e) {
+      if (e & 1u) r *= base;
+      base *= base;
+      e >>= 1;
+    }
+    return (T)r;
   }
 }
 

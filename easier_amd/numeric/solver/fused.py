@@ -25,6 +25,7 @@ import torch
 from ... import _native as nat
 from ...core import module as esr
 from ...core import ops
+from ...core.fuser import BoundTensors
 from ...core.utils import logger
 from ..linsys import Linsys
 from ..preconditioner.jacobi import Jacobi
@@ -70,6 +71,7 @@ class LinsysCSR:
         self.Av = P['Av'].data
         self.n = P['reducer'].n
         self.dtype = self.Av.dtype
+        self.sources = [P['Ae'], P['Av']]
 
     @staticmethod
     def applicable(A, x: esr.Tensor, prog) -> Optional[str]:
@@ -166,6 +168,7 @@ class FusedCG:
         self.cg = cg
         self.op = LinsysCSR(op if op is not None else cg.A)
         self.diag = diag.data if diag is not None else None
+        self.bound = BoundTensors(self.op.sources + [diag, cg.b, cg.x, cg.r, cg.z, cg.p])
         dev = cg.x.device
         n = self.op.n
         sb, wb = ctypes.c_int64(), ctypes.c_int64()
@@ -201,6 +204,7 @@ class FusedCG:
         """Init (cg.py:25-29) and the exit tolerance on the device; the
         chunk graph is captured on first use."""
         c = self.cg
+        self.bound.check()
         nat.call('esr_pcg_init', self.dt, self.op.n, *self._arrays(),
                  c.b.data.data_ptr(), c.x.data.data_ptr(), c.r.data.data_ptr(),
                  c.z.data.data_ptr(), c.p.data.data_ptr(), self.state.data_ptr(),
@@ -258,6 +262,7 @@ class FusedGMRES:
         self.gm = gm
         self.op = LinsysCSR(op if op is not None else gm.A)
         self.diag = diag.data if diag is not None else None
+        self.bound = BoundTensors(self.op.sources + [diag, gm.b, gm.x, gm.r])
         dev = gm.x.device
         n, m = self.op.n, gm.restart
         sb, wb = ctypes.c_int64(), ctypes.c_int64()
@@ -295,6 +300,7 @@ class FusedGMRES:
         g = self.gm
         name = g.__class__.__name__
         n, m = self.op.n, g.restart
+        self.bound.check()
         if self.graph is None:
             self.graph = _Graph(self._cycle)
         nat.call('esr_gmres_begin', self.dt, n, g.b.data.data_ptr(),

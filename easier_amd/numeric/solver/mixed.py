@@ -22,6 +22,7 @@ from torch import nn
 from ... import _native as nat
 from ...core import module as esr
 from ...core import ops
+from ...core.fuser import BoundTensors
 from ...core.utils import logger
 from .fused import LinsysCSR, _Graph, _precond_diag
 
@@ -83,6 +84,7 @@ class MixedPrecisionCG(esr.Module):
         n = op.n
         f32 = lambda t: None if t is None else t.to(torch.float32)
         self.easier_state = s = dict(op=op, n=n, diag64=diag)
+        s['bound'] = BoundTensors(op.sources + [self.b, self.x])
         s['Ae32'], s['Av32'], s['diag32'] = f32(op.Ae), f32(op.Av), f32(diag)
         for k in ('r32', 'd32', 'ri', 'z', 'p', 'Ap'):
             s[k] = torch.zeros(n, dtype=torch.float32, device=dev)
@@ -142,6 +144,7 @@ class MixedPrecisionCG(esr.Module):
         if self.easier_state is None:
             raise RuntimeError('run esr.compile() first')
         s = self.easier_state
+        s['bound'].check()
         if s['graph'] is None or s.get('chunk') != chunk:
             s['graph'] = _Graph(lambda sp: self._inner_steps(chunk, sp))
             s['chunk'] = chunk

@@ -276,16 +276,21 @@ def test_codegen_options_are_bit_identical(gpu, dtype, option):
         assert torch.equal(outs[1][0], ref.h)
 
 
-def test_compile_cache_round_trip(gpu, tmp_path):
+@pytest.mark.parametrize('reorder,freeze', [('rcm', False), ('blocks', True)])
+def test_compile_cache_round_trip(gpu, tmp_path, reorder, freeze):
     """esr.dump() then esr.compile(load_dir=...) on a fresh program: the same
-    renumbering, the same SWE steps bit for bit."""
-    mesh = wmesh.tri2(12)
+    renumbering, and the SWE steps of both compiles equal the oracle's bit
+    for bit (float64)."""
+    mesh = wmesh.tri2(12) if reorder == 'rcm' else wmesh.tri4(24)
     geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
     dt = 0.2 * wswe.cfl_dt(mesh, geo['h'], 9.81)
+    ref = oswe.SWE(mesh, geo, dt=dt, g=9.81)
+    for _ in range(5):
+        ref()
     states = []
     for load in (None, str(tmp_path)):
         eqn = wswe.ShallowWaterEquation(mesh, geo, dt=dt, g=9.81)
-        esr.compile([eqn], 'hip', load_dir=load)
+        esr.compile([eqn], 'hip', load_dir=load, reorder=reorder, freeze=freeze)
         if load is None:
             esr.dump([eqn], str(tmp_path))
         for _ in range(5):
@@ -293,8 +298,9 @@ def test_compile_cache_round_trip(gpu, tmp_path):
         states.append([eqn.h.collect(), eqn.uh.collect(), eqn.vh.collect(),
                        {eqn.easier_program.space_name[k]: v
                         for k, v in eqn.easier_program.space_order.items()}])
-    for a, b in zip(states[0][:3], states[1][:3]):
+    for a, b, k in zip(states[0][:3], states[1][:3], ('h', 'uh', 'vh')):
         assert torch.equal(a, b)
+        assert torch.equal(a, getattr(ref, k)), k
     assert states[0][3].keys() == states[1][3].keys()
     for k in states[0][3]:
         assert np.array_equal(states[0][3][k], states[1][3][k])
@@ -388,3 +394,39 @@ def test_recomputed_gathers_match_the_node_engine(gpu):
         outs.append((m.x.collect(), m.u.collect()))
     assert torch.equal(outs[0][0], outs[1][0])
     assert torch.equal(outs[0][1], outs[1][1])
+
+
+def test_replaced_storage_is_refused(gpu):
+    """Compiled kernels and captured graphs hold device addresses: giving a
+    compiled tensor new storage (t.data = ...) after compile() raises at the
+    next launch instead of computing on the old storage; in-place updates
+    (t.data.copy_) are seen."""
+    eqn = Poisson5(12, seed=1)
+    sol = CG(eqn.A, eqn.b, eqn.x, Jacobi(eqn.Ac))
+    esr.compile([eqn, sol], 'hip')
+    assert sol.easier_fused is not None
+    sol.solve(rtol=1e-8, maxiter=500)
+    eqn.x.data.copy_(torch.zeros_like(eqn.x.data))
+    info = sol.solve(rtol=1e-8, maxiter=500)
+    assert info['residual'] <= 1e-8 * float(eqn.b.data.norm())
+    old = eqn.x.data
+    eqn.x.data = torch.zeros_like(old)
+    with pytest.raises(RuntimeError, match='storage of a compiled tensor'):
+        sol.solve(rtol=1e-8, maxiter=500)
+    eqn.x.data = old
+    sol.solve(rtol=1e-8, maxiter=500)
+
+    mesh = wmesh.tri2(8)
+    geo = oswe.geometry(mesh, centre=(0.45, 0.55), width=200.0, amp=0.1)
+    swe = wswe.ShallowWaterEquation(mesh, geo, dt=0.2 * wswe.cfl_dt(mesh, geo['h'], 9.81),
+                                    g=9.81)
+    esr.compile([swe], 'hip')
+    assert type(swe.easier_engine).__name__ == 'FusedEngine'
+    swe()
+    swe()                                   # now a graph replay
+    h = swe.h.data
+    swe.h.data = h.clone()
+    with pytest.raises(RuntimeError, match='storage of a compiled tensor'):
+        swe()
+    swe.h.data = h
+    swe()

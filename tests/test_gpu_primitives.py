@@ -253,3 +253,19 @@ def test_primitives_at_full_size(gpu, dtype):
     out = ops.csr_reduce(got.to(gpu), rowptr.to(gpu), None, lr.to(gpu), nc, 'sum').cpu()
     ref = oops.reducer(want, dst, nc)
     assert torch.equal(out, ref)
+
+
+def test_ewise_integer_pow(gpu):
+    """Integer powers follow torch's integer arithmetic: a negative exponent
+    gives 1 for a = 1, +-1 for a = -1 and 0 otherwise; large exponents wrap
+    modulo 2^64; a negative scalar exponent is an error."""
+    a = torch.tensor([2, 1, -1, 0, 3, -2, -1, 7, 3, 5], dtype=torch.int64)
+    b = torch.tensor([-1, -2, -3, -1, 0, 63, 64, 3, 2 ** 40, 2 ** 62 + 1], dtype=torch.int64)
+    got = ops.ewise('pow', a.to(gpu), b.to(gpu)).cpu()
+    assert torch.equal(got, torch.pow(a, b)), (got, torch.pow(a, b))
+    assert int(got[8]) == -7860764868738023423
+    a32 = torch.arange(-20, 20, dtype=torch.int32)
+    for e in (0, 1, 2, 3, 7, 31):
+        assert torch.equal(ops.ewise('pow', a32.to(gpu), e).cpu(), a32 ** e), e
+    with pytest.raises(RuntimeError, match='negative integer powers'):
+        ops.ewise('pow', a.to(gpu), -2)
