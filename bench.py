@@ -175,6 +175,42 @@ def pcg_iteration_roofline(cg, n, nnz, T, chunks=3):
                 avg_iteration_s=it)
 
 
+def committed_pcg_traffic(dtype):
+    """HBM bytes of one Jacobi-PCG iteration (its three kernels) on the
+    2048 x 2048 variable-coefficient system from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary in profiles/ (line 'traffic_bytes iteration <bytes>'),
+    or None."""
+    path = os.path.join(REPO, 'profiles', f'pcg_varcoef_2048_{dtype}_traffic.md')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[:2] == ['traffic_bytes', 'iteration']:
+                return int(parts[2])
+    return None
+
+
+def pcg_profile(dtype):
+    """The Jacobi-PCG iteration alone: the 2048 x 2048 variable-coefficient
+    system (seed 0, the mixed-precision workload's) compiled in `dtype`,
+    pcg_iteration_roofline over 3 chunks."""
+    import torch
+    import easier_amd as esr
+    from easier_amd.numeric.solver import CG
+    from easier_amd.workloads import poisson as wp
+    _, sys_, _ = wp.varcoef_system(2048, seed=0)
+    e1 = wp.PoissonP1(sys_, dtype=torch.float64 if dtype == 'f64' else torch.float32)
+    cg = CG(e1.A, e1.b, e1.x, e1.M)
+    esr.compile([e1, cg], 'hip')
+    T = 8 if dtype == 'f64' else 4
+    roof = pcg_iteration_roofline(cg, int(sys_['Av'].shape[0]), int(sys_['row'].shape[0]), T)
+    roof['traffic'] = committed_pcg_traffic(dtype)
+    torch.cuda.synchronize()
+    return dict(workload=f'Jacobi-PCG iteration, variable-coefficient P1 Poisson 2048x2048 '
+                f'(4,198,401 vertices), {dtype}', roofline=roof)
+
+
 def poisson_config1(skip_cpu):
     import torch
     import easier_amd as esr
@@ -459,6 +495,7 @@ def mixed_config4(reps=1, skip_cpu=False):
                           reset=lambda: e1.x.data.zero_())
     x64 = e1.x.collect().numpy()
     roof = pcg_iteration_roofline(cg, int(sys_['Av'].shape[0]), int(sys_['row'].shape[0]), 8)
+    roof['traffic'] = committed_pcg_traffic('f64')
     log(f'config 4 fp64 Jacobi-PCG iteration: {roof["avg_iteration_s"] * 1e6:.1f} us, '
         f'{roof["frac"]:.1%} of 8 TB/s')
     e2 = wp.PoissonP1(sys_)
@@ -503,6 +540,26 @@ def mixed_config4(reps=1, skip_cpu=False):
     return out
 
 
+def gnn_application_roofline(app):
+    """Roofline of one network application z = M(r) (||r||, the encoder and
+    the L message-passing layers: csrc/k_gnn.hip), from its event-timed
+    duration. Algorithmic bytes, each array once: ||r|| reads r (8 n); the
+    encoder reads r, the vertex features (8 n each) and writes h and Q
+    (bfloat16, 2 H n each, H = 32); a layer reads rowptr (4 n), h, Q (2 H n
+    each; Q is gathered, counted once), col and the edge features (16 per
+    edge) and writes the next h and Q (2 H n each); the last layer writes z
+    and reads r and d instead of writing h and Q (24 n)."""
+    n, nnz, L, H = app['vertices'], app['edges'], app['layers'], 32
+    layer_in = n * (4 + 2 * 2 * H) + nnz * 16
+    bytes_ = n * 8 + n * (16 + 2 * 2 * H) + (L - 1) * (layer_in + n * 2 * 2 * H) + \
+        layer_in + n * 24
+    t = app['us_per_application'] * 1e-6
+    return dict(bound='hbm', kernel=f'network application (k_gnn_encode + {L} x k_gnn_layer)',
+                achieved=bytes_ / t / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                frac=bytes_ / t / HBM_PEAK, traffic=None, bytes_per_launch=bytes_,
+                avg_application_s=t)
+
+
 def learned_config5(weights, seeds=(1,), reps=1,
                     methods=('jacobi', 'mg', 'mg-learned'), skip_cpu=False):
     """Learned-preconditioner flexible CG against Jacobi-PCG on unseen
@@ -519,6 +576,7 @@ def learned_config5(weights, seeds=(1,), reps=1,
         res = wl.evaluate_compiled(pre, 2048, s, rtol=1e-8, reps=reps, maxiter=50000,
                                    methods=methods)
         res['network_application'] = wl.time_forward(pre, 2048, s, reps=20)
+        res['network_roofline'] = gnn_application_roofline(res['network_application'])
         if not skip_cpu and 'jacobi' in res:
             rate = cpu_pcg_rate(wp.varcoef_system(2048, s)[1])
             res['cpu_jacobi_s_per_iter'] = rate
@@ -577,6 +635,10 @@ def main():
                          'empty for none')
     ap.add_argument('--weights', default=os.path.join(REPO, 'easier_amd', 'workloads',
                                                       'gnn_smoother_l3_h32.npz'))
+    ap.add_argument('--pcg-profile', action='store_true',
+                    help='only the Jacobi-PCG iteration of the 2048 x 2048 variable-'
+                         'coefficient system in --dtype, 3 captured chunks of 32 '
+                         'iterations after one warm-up chunk (for rocprofv3 runs)')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy a float64 array of MIB MiB 5 times (the known-size '
                          'kernel that scales the HBM byte counters)')
@@ -591,6 +653,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.pcg_profile:
+        print(json.dumps(pcg_profile(args.dtype)), flush=True)
         return
     dtype = torch.float64 if args.dtype == 'f64' else torch.float32
     T = 8 if dtype == torch.float64 else 4

@@ -454,8 +454,8 @@ def time_forward(pre: LearnedPreconditioner, n, seed=1, reps=50):
         pre.forward_into(prob.graph, bufs, r, z)
     b.record()
     torch.cuda.synchronize()
-    return dict(n_grid=n, vertices=prob.n, seed=seed, reps=reps,
-                us_per_application=a.elapsed_time(b) * 1e3 / reps)
+    return dict(n_grid=n, vertices=prob.n, edges=int(prob.col.shape[0]), layers=pre.L,
+                seed=seed, reps=reps, us_per_application=a.elapsed_time(b) * 1e3 / reps)
 
 
 def main(argv=None):
