@@ -11,6 +11,10 @@
 // Solver scalars stay on the device; every kernel of an iteration is a
 // no-op once the loop has exited, so chunks of iterations can be captured
 // in a HIP graph and replayed until the host sees `done`.
+#include <algorithm>
+#include <type_traits>
+#include <utility>
+
 #include "esr_common.h"
 #include "esr_krylov.h"
 #include "esr_solver.h"
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(kRB) void k_gm_residual_fin(int64_t g,
 // InitV/UpdateV + InitW (gmres.py:48, 64, 120): V_j = src / s and
 // w = A(M(V_j)), M = identity or r / diag. For j = 0, s = ||r||. For j > 0,
 // s = H[j, j-1] = ||w|| from the previous norm kernel's partials (stored in
-// H, gmres.py:90, 210-213); below 1e-15 the cycle breaks down at j - 1. The
+// H, gmres.py:90, 210-213; g_in of them); below 1e-15 the cycle breaks down at j - 1. The
 // breakdown flag is read and (by workgroup 0) written in this kernel, so both
 // accesses are atomic; every workgroup derives the same decision either way.
 template <typename T>
@@ -354,14 +358,14 @@ __global__ __launch_bounds__(kRB) void k_gm_spmv(
     int64_t n, int j, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ diag,
     const T *__restrict__ src, T *__restrict__ Vj, T *__restrict__ w, GmresState *st,
-    const double *__restrict__ part_in) {
+    const double *__restrict__ part_in, int64_t g_in) {
   if (__hip_atomic_load(&st->broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   T s;
   if (j == 0) {
     s = (T)st->rnorm;
   } else {
     double tot[1];
-    sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
+    sum_partials<1>(part_in, g_in, 0, g_in, tot);
     const double hn = (double)(T)cr_sqrt(tot[0]);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->H[j * kGmresMaxRestart + (j - 1)] = hn;
@@ -382,56 +386,156 @@ __global__ __launch_bounds__(kRB) void k_gm_spmv(
   }
 }
 
-// Modified Gram-Schmidt step i of Arnoldi step j (gmres.py:79, 105):
-// H[i-1, j] from the previous projection's partials, w -= H[i-1, j] V_{i-1},
-// then partials of V_i . w.
-template <typename T>
-__global__ __launch_bounds__(kRB) void k_gm_mgs(int64_t n, int i, int j,
-                                               const T *__restrict__ Vprev,
-                                               const T *__restrict__ Vi, T *__restrict__ w,
-                                               GmresState *st, const double *__restrict__ part_in,
-                                               double *__restrict__ part_out) {
-  if (st->broken) return;
-  T hprev = T(0);
-  if (i > 0) {
-    double tot[1];
-    sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
-    const double h = (double)(T)tot[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * kGmresMaxRestart + j] = h;
-    hprev = (T)h;
-  }
-  double acc[1] = {0.0};
-  ESR_GRID_ROWS(r, n) {
-    T wr = w[r];
-    if (i > 0) {
-      wr = wr - hprev * Vprev[r];
-      w[r] = wr;
-    }
-    acc[0] += (double)(Vi[r] * wr);
-  }
-  store_partials<1>(acc, part_out);
+// Orthogonalisation of w against V_0..V_j (gmres.py:79-90 computes
+// H[i, j] = V_i . w and w -= H[i, j] V_i one i at a time, modified
+// Gram-Schmidt). Here: classical Gram-Schmidt applied twice (CGS2), which
+// keeps the basis as orthogonal as MGS does (Giraud, Langou, Rozloznik and
+// van den Eshof, Numer. Math. 101, 2005) with three grid-wide reductions per
+// Arnoldi step instead of j + 2, each one kernel:
+//   k_gm_dots:     partials of h_i = V_i . w, i <= j
+//   k_gm_cgs<1>:   h_i from them, w -= sum_i h_i V_i, partials of g_i = V_i . w
+//   k_gm_cgs<0>:   g_i, w -= sum_i g_i V_i, H[i, j] = h_i + g_i, partials of w . w
+// (the next step's k_gm_spmv finishes ||w|| = H[j+1, j]). In exact
+// arithmetic both give the same H and basis. The kernels run on a grid of at
+// most kGsMaxGrid workgroups so that re-summing (j + 1) rows of partials in
+// every workgroup stays cheap. JB is the register block of basis vectors per
+// row (8, 16 or 32 >= j + 1).
+constexpr int kGsMaxGrid = 256;
+inline unsigned gs_grid(int64_t n) {
+  const int64_t g = ceil_div(n, kRB);
+  return (unsigned)(g < kGsMaxGrid ? (g > 0 ? g : 1) : kGsMaxGrid);
 }
 
-// Last MGS update (gmres.py:90): H[j, j] from the projection's partials,
-// w -= H[j, j] V_j, then partials of w.w (NormW, gmres.py:210-213).
-template <typename T>
-__global__ __launch_bounds__(kRB) void k_gm_norm(int64_t n, int j, const T *__restrict__ Vj,
-                                                T *__restrict__ w, GmresState *st,
-                                                const double *__restrict__ part_in,
+// Workgroup sums of per-thread values acc[i] (i < cnt), stored to
+// part[i * gridDim.x + blockIdx.x]. Within a wavefront the JB sums are
+// formed together by halving exchanges: at offset 32 a lane keeps one half
+// of its values and adds its partner's copy of that half (lanes with bit 5
+// set keep the upper half), then offset 16 on the remaining half, ... until
+// one value is left, which is value lane / (64 / JB) summed over 64 / (64 /
+// JB) lanes; xor exchanges over the remaining low offsets finish it. JB - 1
+// + log2(64 / JB) exchanges instead of 6 per value; a fixed order, so every
+// run gives the same bits.
+template <int JB>
+__device__ __forceinline__ void gs_store_partials(double (&acc)[JB], int cnt,
+                                                  double *__restrict__ part) {
+  static_assert(JB >= 1 && JB <= 32 && (JB & (JB - 1)) == 0, "JB: a power of 2 up to 32");
+  __shared__ double sh[JB][kRB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int kLog = JB >= 32 ? 5 : JB >= 16 ? 4 : JB >= 8 ? 3 : JB >= 4 ? 2 : JB >= 2 ? 1 : 0;
+#pragma unroll
+  for (int step = 0; step < kLog; ++step) {
+    const int m = JB >> (step + 1), off = 32 >> step;
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int k = 0; k < m; ++k) {
+      // select between values, not between array slots: a select of the
+      // two slots' addresses would put acc in scratch memory
+      const double lo = acc[k], hi = acc[k + m];
+      const double send = upper ? lo : hi;
+      const double keep = upper ? hi : lo;
+      acc[k] = keep + __shfl_xor(send, off, 64);
+    }
+  }
+#pragma unroll
+  for (int off = 32 >> kLog; off >= 1; off >>= 1) acc[0] += __shfl_xor(acc[0], off, 64);
+  constexpr int span = 64 / JB;  // lanes that hold the same sum
+  if (lane % span == 0) sh[lane / span][wid] = acc[0];
+  __syncthreads();
+  if ((int)threadIdx.x < cnt) {
+    double t = 0.0;
+    for (int w = 0; w < kRB / 64; ++w) t += sh[threadIdx.x][w];
+    part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// out[i] = (T) sum_{b < g} part[i * g + b] for i < cnt (g <= kGsMaxGrid),
+// in every workgroup: wavefront w sums rows i = w, w + 4, ...; each lane
+// adds partials b = lane, lane + 64, ... in that order, then the shuffle
+// tree (a fixed order, the same in every workgroup). All loads (clamped
+// positions) are issued before the first sum.
+template <typename T, int JB>
+__device__ __forceinline__ void gs_sum_rows(const double *__restrict__ part, int64_t g, int cnt,
+                                            T *out) {
+  constexpr int RW = (JB + kRB / 64 - 1) / (kRB / 64);  // rows per wavefront
+  constexpr int KB = kGsMaxGrid / 64;                   // partials per lane and row
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double v[RW][KB];
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+    const int i = wid + (kRB / 64) * q;
+    const int64_t ic = i < cnt ? i : cnt - 1;
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int64_t b = lane + 64 * k;
+      v[q][k] = part[ic * g + (b < g ? b : g - 1)];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < KB; ++k)
+      if (lane + 64 * k < g) t += v[q][k];
+    t = kr_wave_sum(t);
+    const int i = wid + (kRB / 64) * q;
+    if (lane == 0 && i < cnt) out[i] = (T)t;
+  }
+  __syncthreads();
+}
+
+template <typename T, int JB>
+__global__ __launch_bounds__(kRB) void k_gm_dots(int64_t n, int j, const T *__restrict__ V,
+                                                const T *__restrict__ w, const GmresState *st,
                                                 double *__restrict__ part_out) {
   if (st->broken) return;
-  double tot[1];
-  sum_partials<1>(part_in, gridDim.x, 0, gridDim.x, tot);
-  const double hd = (double)(T)tot[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->H[j * kGmresMaxRestart + j] = hd;
-  const T h = (T)hd;
-  double acc[1] = {0.0};
+  double acc[JB];
+#pragma unroll
+  for (int i = 0; i < JB; ++i) acc[i] = 0.0;
   ESR_GRID_ROWS(r, n) {
-    const T wr = w[r] - h * Vj[r];
-    w[r] = wr;
-    acc[0] += (double)(wr * wr);
+    const T wr = w[r];
+    // every load unconditional (rows past j clamped to row j), so all of
+    // them are in flight together; the sums past j are never stored
+#pragma unroll
+    for (int i = 0; i < JB; ++i)
+      acc[i] += (double)(V[(int64_t)(i <= j ? i : j) * n + r] * wr);
   }
-  store_partials<1>(acc, part_out);
+  gs_store_partials<JB>(acc, j + 1, part_out);
+}
+
+template <typename T, int JB, bool FIRST>
+__global__ __launch_bounds__(kRB) void k_gm_cgs(int64_t n, int j, const T *__restrict__ V,
+                                               T *__restrict__ w, GmresState *st,
+                                               const double *__restrict__ part_in,
+                                               double *__restrict__ part_out) {
+  if (st->broken) return;
+  __shared__ T h[JB];
+  gs_sum_rows<T, JB>(part_in, gridDim.x, j + 1, h);
+  if (blockIdx.x == 0 && (int)threadIdx.x <= j) {
+    double *Hij = &st->H[threadIdx.x * kGmresMaxRestart + j];
+    // pass 1 stores h_i; pass 2 completes H[i, j] = h_i + g_i
+    *Hij = FIRST ? (double)h[threadIdx.x] : (double)((T)*Hij + h[threadIdx.x]);
+  }
+  double acc[FIRST ? JB : 1];
+#pragma unroll
+  for (int i = 0; i < (FIRST ? JB : 1); ++i) acc[i] = 0.0;
+  ESR_GRID_ROWS(r, n) {
+    T v[JB];
+#pragma unroll
+    for (int i = 0; i < JB; ++i) v[i] = V[(int64_t)(i <= j ? i : j) * n + r];
+    T wr = w[r];
+#pragma unroll
+    for (int i = 0; i < JB; ++i)
+      if (i <= j) wr = wr - h[i] * v[i];
+    w[r] = wr;
+    if (FIRST) {
+#pragma unroll
+      for (int i = 0; i < JB; ++i)
+        if (i <= j) acc[i] += (double)(v[i] * wr);
+    } else {
+      acc[0] += (double)(wr * wr);
+    }
+  }
+  gs_store_partials<FIRST ? JB : 1>(acc, FIRST ? j + 1 : 1, part_out);
 }
 
 // End of a cycle of m steps: H[m, m-1] = ||w|| from the last norm's partials.
@@ -667,15 +771,20 @@ ESR_API int esr_pcg_read(const void *state, double *scalars, int64_t *ints, void
   return ESR_OK;
 }
 
-// Workspace of the GMRES kernels: two regions of G partials used
-// alternately by consecutive reductions.
+// Workspace of the GMRES kernels: one region of red_grid(n) partials (the
+// norms of b and r) and two regions of kGmresMaxRestart rows of gs_grid(n)
+// partials, used alternately by the Gram-Schmidt reductions.
+static int64_t gmres_ws_doubles(int64_t n) {
+  return std::max<int64_t>((int64_t)red_grid(n), 2 * (int64_t)kGmresMaxRestart * gs_grid(n));
+}
+
 ESR_API int esr_gmres_workspace(int64_t n, int64_t restart, int64_t *state_bytes,
                                 int64_t *ws_bytes) {
   ESR_REQUIRE(n >= 1 && restart >= 1 && state_bytes && ws_bytes, "esr_gmres_workspace: bad");
   ESR_REQUIRE(restart <= kGmresMaxRestart, "esr_gmres_workspace: restart %lld > %d",
               (long long)restart, kGmresMaxRestart);
   *state_bytes = (int64_t)sizeof(GmresState);
-  *ws_bytes = 2 * (int64_t)red_grid(n) * (int64_t)sizeof(double);
+  *ws_bytes = gmres_ws_doubles(n) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -727,29 +836,31 @@ ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t
   ESR_REQUIRE(rowptr && col && Ae && Av && r && V && w0 && w1 && state && ws,
               "esr_gmres_cycle: null argument");
   hipStream_t s = as_stream(stream);
-  const unsigned g = red_grid(n);
-  double *reg[2] = {(double *)ws, (double *)ws + g};
+  const unsigned g = red_grid(n), gg = gs_grid(n);
+  double *reg[2] = {(double *)ws, (double *)ws + (int64_t)kGmresMaxRestart * gg};
   ESR_FLOAT_DISPATCH(dtype, {
     T *Vt = (T *)V;
     T *w[2] = {(T *)w0, (T *)w1};
     GmresState *st = (GmresState *)state;
-    int last = 0;  // region the latest reduction wrote (the residual: 0)
     for (int j = 0; j < (int)restart; ++j) {
       const T *src = j == 0 ? (const T *)r : w[(j - 1) & 1];
       T *wj = w[j & 1];
+      // the norm partials of w_{j-1} are in reg[0] (written by k_gm_cgs<0>)
       k_gm_spmv<T><<<g, kRB, 0, s>>>(n, j, rowptr, col, (const T *)Ae, (const T *)Av,
                                      (const T *)diag, src, Vt + (int64_t)j * n, wj, st,
-                                     reg[last]);
-      for (int i = 0; i <= j; ++i) {
-        k_gm_mgs<T><<<g, kRB, 0, s>>>(n, i, j, i > 0 ? Vt + (int64_t)(i - 1) * n : Vt,
-                                      Vt + (int64_t)i * n, wj, st, reg[last], reg[1 - last]);
-        last = 1 - last;
-      }
-      k_gm_norm<T><<<g, kRB, 0, s>>>(n, j, Vt + (int64_t)j * n, wj, st, reg[last],
-                                     reg[1 - last]);
-      last = 1 - last;
+                                     reg[0], (int64_t)gg);
+      auto orth = [&](auto jb) {
+        constexpr int JB = decltype(jb)::value;
+        k_gm_dots<T, JB><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[1]);
+        k_gm_cgs<T, JB, true><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[1], reg[0]);
+        k_gm_cgs<T, JB, false><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[0], reg[1]);
+      };
+      if (j < 8) orth(std::integral_constant<int, 8>());
+      else if (j < 16) orth(std::integral_constant<int, 16>());
+      else orth(std::integral_constant<int, 32>());
+      std::swap(reg[0], reg[1]);   // the norm partials now in reg[0]
     }
-    k_gm_fin<T><<<1, kRB, 0, s>>>(g, (int)restart, st, reg[last]);
+    k_gm_fin<T><<<1, kRB, 0, s>>>(gg, (int)restart, st, reg[0]);
   });
   ESR_LAUNCHED();
   return ESR_OK;

@@ -13,8 +13,10 @@ CG: one init kernel, then chunks of iterations, each iteration three kernels
     per chunk. Iterations after the exit test are no-ops, so `iters`,
     `residual` and x are those of cg.py:106-137.
 GMRES: per restart cycle one residual kernel, the whole Arnoldi process as one
-    captured graph (SpMV, one kernel per MGS projection, norm), one host read of
-    H, the small SVD least-squares solve on the host, one update kernel.
+    captured graph (per step: SpMV, then w orthogonalised by classical
+    Gram-Schmidt applied twice -- three reductions, one kernel each, where the
+    reference's modified Gram-Schmidt takes j + 2), one host read of H, the
+    small SVD least-squares solve on the host, one update kernel.
 """
 import ctypes
 from typing import Optional

@@ -297,8 +297,10 @@ int esr_pcg_read(const void *state, double *scalars, int64_t *ints,
 /* ---- a6: fused GMRES(m) Arnoldi cycle ---------------------------------------
  * easier/numeric/solver/gmres.py:12-266 (m <= 32). esr_gmres_begin: ||b||;
  * esr_gmres_residual: r = b - A x, ||r|| (UpdateRnorm); esr_gmres_cycle:
- * V_0 = r/||r||, then for j < m: w = A(M(V_j)), modified Gram-Schmidt against
- * V_0..V_j into H[:, j], H[j+1, j] = ||w||, V_{j+1} = w / H[j+1, j], stopping
+ * V_0 = r/||r||, then for j < m: w = A(M(V_j)), orthogonalised against
+ * V_0..V_j into H[:, j] (the reference's modified Gram-Schmidt, here classical
+ * Gram-Schmidt applied twice: three reductions per step), H[j+1, j] = ||w||,
+ * V_{j+1} = w / H[j+1, j], stopping
  * (later kernels no-op) at H[j+1, j] < 1e-15; V is m x n (one basis vector per
  * row), w0/w1 scratch n-vectors. esr_gmres_read: scalars {rnorm, bnorm}, ints
  * {jend, broken}, H ((m+1) x m row-major). esr_gmres_update_x:
