@@ -8,6 +8,7 @@ partition, so a loader only ever loads fully, onto the host, and the compiler
 permutes and uploads the result.
 """
 import copy
+import os
 from typing import Optional, Tuple, Union
 
 import numpy as np
@@ -112,32 +113,37 @@ class ArangeTensorLoader(DataLoaderBase):
 
 
 class H5DataLoader(DataLoaderBase):
-    """A dataset in an HDF5 file (the mesh schema of
-    tutorial/create_triangular_mesh.py:29-35). Reading needs h5py; the shape
-    and dtype are read when the loader is created."""
+    """A dataset in an HDF5 file (data_loader.py:418-751; e.g. the mesh schema
+    of tutorial/create_triangular_mesh.py:29-35), read with the package's own
+    HDF5 reader (core/h5.py: contiguous or compact datasets of integers or
+    floating point, as h5py writes them by default) -- h5py is not needed.
+    Shape and dtype are read when the loader is created; the values when the
+    program is compiled. h5_file_kwargs (h5py.File options in the reference)
+    are accepted and ignored."""
 
     def __init__(self, file: str, dataset: str, dtype=None, device='cpu',
                  **h5_file_kwargs) -> None:
-        try:
-            import h5py  # noqa: F401
-        except ImportError as e:
-            raise ImportError(
-                'esr.hdf5 needs h5py, which is not installed; load the data '
-                'with numpy/torch and pass the tensor instead') from e
-        import h5py
-        self.file, self.dataset = file, dataset
+        from . import h5
+        self.file, self.dataset = os.path.expanduser(file), dataset
         self.h5_file_kwargs = h5_file_kwargs
-        with h5py.File(file, 'r', **h5_file_kwargs) as f:
-            d = f[dataset]
-            self.shape = tuple(d.shape)
-            raw = torch.from_numpy(np.empty((0,), dtype=d.dtype)).dtype
+        with h5.H5Reader(self.file) as r:
+            d = r.dataset(dataset)
+        self.shape = tuple(d.shape)
+        raw = torch.from_numpy(np.empty((0,), dtype=d.dtype.newbyteorder('='))).dtype
         self.dtype = dtype if dtype is not None else raw
         self.device = torch.device(device)
 
+    def load_rows(self, start: int, stop: int) -> torch.Tensor:
+        """Rows [start, stop) only (partially_load_by_chunk's unit, :572-590)."""
+        from . import h5
+        with h5.H5Reader(self.file) as r:
+            arr = r.read(self.dataset, start, stop)
+        return torch.from_numpy(np.ascontiguousarray(arr)).to(self.dtype)
+
     def load_cpu(self):
-        import h5py
-        with h5py.File(self.file, 'r', **self.h5_file_kwargs) as f:
-            arr = f[self.dataset][...]
+        from . import h5
+        with h5.H5Reader(self.file) as r:
+            arr = r.read(self.dataset)
         return torch.from_numpy(np.ascontiguousarray(arr)).to(self.dtype)
 
     def __repr__(self):

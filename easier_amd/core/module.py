@@ -275,20 +275,19 @@ class Tensor(nn.Parameter):
         return host.to(self.easier_data_loader.device, copy=True)
 
     def save(self, h5_file_path, h5_dataset_path, **h5_file_kwargs) -> None:
+        """Add the tensor (in the user's order) to an HDF5 file as a new
+        dataset, creating the file if needed (module.py:614-707, h5py mode
+        'a'), with the package's own HDF5 writer (core/h5.py); an existing
+        dataset of that name raises. h5_file_kwargs are accepted and ignored."""
         if not self.easier_data_ready:
             raise RuntimeError("Tensor data is not ready, run compile() first")
         import os
-        try:
-            import h5py
-        except ImportError as e:
-            raise ImportError('Tensor.save needs h5py, which is not '
-                              'installed') from e
+        from . import h5
         path = os.path.expanduser(h5_file_path)
         d = os.path.dirname(path)
         if d:
             os.makedirs(d, exist_ok=True)
-        with h5py.File(path, mode='a', **h5_file_kwargs) as f:
-            f.create_dataset(h5_dataset_path, data=self.collect().cpu().numpy())
+        h5.append(path, {h5_dataset_path: self.collect().cpu().numpy()})
 
 
 # ---------------------------------------------------------------------------

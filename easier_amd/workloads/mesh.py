@@ -42,3 +42,24 @@ def tri2(n: int, jitter: float = 0.0, seed: int = 0):
     nat.call('esr_mesh_tri2', n, float(jitter), int(seed),
              *[m[k].ctypes.data for k in _ORDER])
     return m
+
+
+def save_h5(mesh, path: str) -> None:
+    """Write a mesh in the schema of tutorial/create_triangular_mesh.py:29-35
+    (datasets src, dst, cells, bcells, points, bpoints) with the package's
+    HDF5 writer; h5py and the reference's loaders read the file."""
+    from ..core import h5
+    h5.write(path, {k: np.asarray(mesh[k]) for k in ('src', 'dst', 'cells', 'bcells',
+                                                      'points', 'bpoints')})
+
+
+def load_h5(path: str):
+    """A mesh file of that schema (as create_triangular_mesh.py writes it with
+    h5py) as the dict tri4/tri2 return: int64 indices, float64 points."""
+    from ..core import h5
+    out = {}
+    with h5.H5Reader(path) as r:
+        for k in _ORDER:
+            a = r.read(k)
+            out[k] = np.ascontiguousarray(a, np.float64 if k == 'points' else np.int64)
+    return out

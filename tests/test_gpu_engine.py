@@ -430,3 +430,53 @@ def test_replaced_storage_is_refused(gpu):
         swe()
     swe.h.data = h
     swe()
+
+
+def test_hdf5_loaders_and_tensor_save(gpu, tmp_path):
+    """A program whose Selector, Reducer and Tensor come from esr.hdf5
+    datasets of a mesh file (the reference's tutorial layout) computes what
+    the same program built from in-memory arrays computes; Tensor.save adds
+    datasets that read back as collect()."""
+    from easier_amd.core import h5
+    mesh = wmesh.tri4(8)
+    path = str(tmp_path / 'mesh.h5')
+    wmesh.save_h5(mesh, path)
+    nc = mesh['cells'].shape[0]
+    g = torch.Generator().manual_seed(11)
+    w = torch.randn(mesh['src'].shape[0], generator=g, dtype=torch.double)
+    h5.append(path, {'w': w.numpy()})
+
+    class P(esr.Module):
+        def __init__(self, from_file):
+            super().__init__()
+            if from_file:
+                src, dst = esr.hdf5(path, 'src'), esr.hdf5(path, 'dst')
+                wt = esr.hdf5(path, 'w')
+            else:
+                src, dst, wt = (torch.from_numpy(mesh['src']), torch.from_numpy(mesh['dst']), w)
+            self.sel = esr.Selector(src)
+            self.red = esr.Reducer(dst, nc)
+            self.w = esr.Tensor(wt, mode='partition')
+            self.v = esr.Tensor(torch.arange(nc, dtype=torch.double) / nc, mode='partition')
+            self.out = esr.Tensor(esr.zeros((nc,), dtype=torch.double), mode='partition')
+
+        def forward(self):
+            self.out[:] = self.red(self.sel(self.v) * self.w) + self.v
+
+    a, b = P(True), P(False)
+    esr.compile([a], 'hip')
+    esr.compile([b], 'hip')
+    a()
+    b()
+    ref = oops.reducer(oops.selector(torch.arange(nc, dtype=torch.double) / nc,
+                                     torch.from_numpy(mesh['src'])) * w,
+                       torch.from_numpy(mesh['dst']), nc) + torch.arange(nc, dtype=torch.double) / nc
+    assert torch.equal(a.out.collect(), b.out.collect())
+    assert torch.equal(a.out.collect(), ref)
+    out_path = str(tmp_path / 'run' / 'out.h5')
+    a.out.save(out_path, 'fields/out')
+    a.v.save(out_path, 'fields/v')
+    assert np.array_equal(h5.read(out_path, 'fields/out'), a.out.collect().numpy())
+    assert np.array_equal(h5.read(out_path, 'fields/v'), a.v.collect().numpy())
+    with pytest.raises(ValueError, match='exists'):
+        a.out.save(out_path, 'fields/out')

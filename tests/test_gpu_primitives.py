@@ -169,6 +169,20 @@ def test_aggregators_one_column(gpu, op, n, dtype):
         assert torch.equal(a, ops.aggregate(op, view.reshape(-1, 1)).cpu().reshape(-1))
 
 
+@pytest.mark.parametrize('op', ['sum', 'max', 'min'])
+@pytest.mark.parametrize('dtype', [torch.int64, torch.int32])
+def test_aggregators_one_column_integers(gpu, op, dtype):
+    """Integer columns long enough for the vector-load stage, aligned and
+    offset by one element: exact."""
+    g = torch.Generator().manual_seed(9)
+    t = torch.randint(-1000, 1000, (300_002,), generator=g, dtype=torch.int64).to(dtype)
+    dev = t.to(gpu)
+    for host, view in ((t[:-1], dev[:-1]), (t[1:], dev[1:])):
+        a = ops.aggregate(op, view.reshape(-1, 1)).cpu().reshape(-1)
+        b = oops.aggregate(op, host.reshape(-1, 1)).reshape(-1)
+        assert int(a[0]) == int(b[0]), (int(a[0]), int(b[0]))
+
+
 @pytest.mark.parametrize('op', ['add', 'sub', 'mul', 'div'])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_ewise_arithmetic_bit_exact(gpu, op, dtype):
