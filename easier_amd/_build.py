@@ -32,7 +32,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ['-O3', '-fPIC', '-std=c++17', '-ffp-contract=off', f'-I{INCLUDE}',
              f'-I{CSRC}', '-Wall', '-Wno-unused-function']
 
-HIP_SOURCES = ['runtime.hip', 'k_basic.hip', 'k_solver.hip', 'k_batched.hip',
+HIP_SOURCES = ['runtime.hip', 'k_basic.hip', 'k_solver.hip', 'k_sell.hip', 'k_batched.hip',
                'k_mrhs.hip', 'k_gnn.hip', 'k_fcg.hip', 'k_mg.hip']
 CXX_SOURCES = ['host_mesh.cpp', 'host_fem.cpp']
 

@@ -136,6 +136,72 @@ __device__ __forceinline__ T linsys_slots(int32_t b, int32_t e, const int32_t *_
   return s;
 }
 
+// ---------------------------------------------------------------------------
+// Sliced ELLPACK with slices of one wavefront (SELL-64; Kreutzer, Hager,
+// Wellein, Fehske and Bishop, SIAM J. Sci. Comput. 36(5), 2014, with sigma =
+// 1: rows keep their order). Rows 64 s .. 64 s + 63 form slice s; its width
+// w_s is the longest of those rows; entry k of row r = 64 s + l sits at
+// soff[s] + 64 k + l (soff[s + 1] - soff[s] = 64 w_s), so each slot is one
+// fully used, contiguous load per wavefront where a CSR row spreads a
+// wavefront's loads over ~6 rows' worth of cache lines per instruction.
+// cnt[r] (uint8) is the row's length; padded entries hold value 0 and a
+// column inside the matrix, and are not folded. Column indices are int16
+// offsets from the row (when every one fits, e.g. a mesh renumbered for
+// locality) or int32 absolute indices. The row is folded exactly like
+// linsys_row: from 0, entries in CSR order, diagonal term last.
+__device__ __forceinline__ int64_t sell_col(int64_t r, const int16_t *__restrict__ c,
+                                            int64_t e) {
+  return r + (int64_t)c[e];
+}
+__device__ __forceinline__ int64_t sell_col(int64_t, const int32_t *__restrict__ c, int64_t e) {
+  return (int64_t)c[e];
+}
+
+// K independent slots (K >= 1), slot positions clamped to the slice width
+// w (every load inside the slice), folded only where k < c.
+template <int K, typename T, typename C, typename Src>
+__device__ __forceinline__ T sell_slots(int64_t r, int64_t e0, int w, int c,
+                                        const C *__restrict__ colv, const T *__restrict__ val,
+                                        Src src) {
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int kk = k < w ? k : w - 1;
+    const int64_t e = e0 + 64 * (int64_t)kk;
+    const T v = src(sell_col(r, colv, e)) * val[e];
+    s = k < c ? s + v : s;
+  }
+  for (int k = K; k < w; ++k) {
+    const int64_t e = e0 + 64 * (int64_t)k;
+    const T v = src(sell_col(r, colv, e)) * val[e];
+    s = k < c ? s + v : s;
+  }
+  return s;
+}
+
+// Row r of the Linsys from its SELL-64 form. Every active lane of the
+// wavefront must hold a row of the same slice (rows visited 64 at a time,
+// wavefront-aligned), so the slice's offset and width are wave-uniform.
+template <typename T, typename C, typename Src>
+__device__ __forceinline__ T sell_row(int64_t r, const int64_t *__restrict__ soff,
+                                      const uint8_t *__restrict__ cnt,
+                                      const C *__restrict__ colv, const T *__restrict__ val,
+                                      const T *__restrict__ Av, Src src) {
+  const int s = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+  const int64_t base = soff[s];
+  const int w = (int)((soff[s + 1] - base) >> 6);
+  const int64_t e0 = base + (r & 63);
+  const int c = cnt[r];
+  T y = T(0);
+  if (w > 6)
+    y = sell_slots<8, T>(r, e0, w, c, colv, val, src);
+  else if (w > 4)
+    y = sell_slots<6, T>(r, e0, w, c, colv, val, src);
+  else if (w > 0)
+    y = sell_slots<4, T>(r, e0, w, c, colv, val, src);
+  return y + src(r) * Av[r];
+}
+
 template <typename T, typename Src>
 __device__ __forceinline__ T linsys_row_wave(int64_t r, const int32_t *__restrict__ rowptr,
                                              const int32_t *__restrict__ col,

@@ -169,6 +169,57 @@ __global__ __launch_bounds__(kRB) void k_pcg_step1(
   store_partials<1>(acc, part_out);
 }
 
+// Kernel 1 on the operator's SELL-64 form (esr_krylov.h): the same
+// decisions, values and fold as k_pcg_step1 (rows are visited 64 at a time,
+// so a wavefront's rows are one slice), bit for bit the same results.
+template <typename T, typename C, bool FUSED>
+__global__ __launch_bounds__(kRB) void k_pcg_step1_sell(
+    int64_t n, const int64_t *__restrict__ soff, const uint8_t *__restrict__ cnt,
+    const C *__restrict__ colv, const T *__restrict__ val, const T *__restrict__ Av,
+    const T *__restrict__ z, T *__restrict__ p0, T *__restrict__ p1, T *__restrict__ Ap,
+    PcgState *st, const double *__restrict__ part_in, double *__restrict__ part_out) {
+  if (st->done) return;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int64_t cur = st->pbuf;
+  const T *__restrict__ pin = cur ? p1 : p0;
+  T *__restrict__ pout = cur ? p0 : p1;
+  double acc[1] = {0.0};
+  if (!FUSED || st->fresh) {
+    if (lead) {
+      st->k_next = st->k_cur;
+      st->rz_next = st->rz_cur;
+      st->pbuf_next = cur;
+    }
+    ESR_GRID_ROWS(i, n) {
+      const T y = sell_row<T>(i, soff, cnt, colv, val, Av, [&](int64_t c) { return pin[c]; });
+      Ap[i] = y;
+      acc[0] += (double)(pin[i] * y);
+    }
+  } else if constexpr (FUSED) {
+    const PcgTest t = pcg_test(part_in, gridDim.x, st->k_cur, st, lead);
+    if (t.exit) return;
+    const T beta = (T)(t.rz / st->rz_cur);
+    if (lead) {
+      st->rr = t.rr;
+      st->rz_new = t.rz;
+      st->rnorm = t.rnorm;
+      st->k_next = t.k + 1;
+      st->rz_next = t.rz;
+      st->beta = (double)beta;
+      st->pbuf_next = 1 - cur;
+    }
+    ESR_GRID_ROWS(i, n) {
+      const T y = sell_row<T>(i, soff, cnt, colv, val, Av,
+                              [&](int64_t c) { return z[c] + beta * pin[c]; });
+      const T pi = z[i] + beta * pin[i];
+      pout[i] = pi;
+      Ap[i] = y;
+      acc[0] += (double)(pi * y);
+    }
+  }
+  store_partials<1>(acc, part_out);
+}
+
 // Kernel 2: Step part 2 (cg.py:47-52) fused with the first half of Update
 // (cg.py:67): alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M r;
 // partials of r.r, r.z. Workgroup 0 commits what kernel 1 decided.
@@ -723,6 +774,54 @@ ESR_API int esr_pcg_steps(int dtype, int64_t n, const int32_t *rowptr, const int
     });
     ESR_LAUNCHED();
   }
+  if (count > 0 && fuse) {
+    ESR_FLOAT_DISPATCH(dtype, (k_pcg_settle<T><<<g, kRB, 0, s>>>(n, (T *)p, (const T *)p1,
+                                                                 (PcgState *)state, part2)));
+    ESR_LAUNCHED();
+  }
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+// esr_pcg_steps with the product on the operator's SELL-64 form (built by
+// esr_sell_plan / esr_sell_fill from the same CSR): the same kernels and
+// results otherwise.
+ESR_API int esr_pcg_steps_sell(int dtype, int64_t n, const int64_t *soff, const uint8_t *cnt,
+                               const void *colv, int32_t idx16, const void *val,
+                               const void *Av, const void *diag, void *x, void *r, void *z,
+                               void *p, void *Ap, void *state, void *ws, int64_t count,
+                               void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && count >= 0 && soff && cnt && colv && val && Av && x && r && z && p &&
+                  Ap && state && ws,
+              "esr_pcg_steps_sell: null argument");
+  hipStream_t s = as_stream(stream);
+  const unsigned g = red_grid(n);
+  const bool fuse = n <= kFuseUpdateMax;
+  const unsigned g1 = fuse ? g : spmv_grid(n);
+  double *part2 = (double *)ws, *part1 = part2 + 2 * (int64_t)g;
+  void *p1 = (char *)ws + pcg_parts_bytes(n);
+#define ESR_STEP1_SELL(C, F)                                                           \
+  k_pcg_step1_sell<T, C, F><<<g1, kRB, 0, s>>>(                                        \
+      n, soff, cnt, (const C *)colv, (const T *)val, (const T *)Av, (const T *)z, (T *)p, \
+      (T *)p1, (T *)Ap, (PcgState *)state, part2, part1)
+  for (int64_t it = 0; it < count; ++it) {
+    ESR_FLOAT_DISPATCH(dtype, {
+      if (fuse) {
+        if (idx16) ESR_STEP1_SELL(int16_t, true); else ESR_STEP1_SELL(int32_t, true);
+      } else {
+        if (idx16) ESR_STEP1_SELL(int16_t, false); else ESR_STEP1_SELL(int32_t, false);
+      }
+      k_pcg_step2<T><<<g, kRB, 0, s>>>(n, (const T *)diag, (T *)x, (T *)r, (T *)z,
+                                       (const T *)p, (const T *)p1, (const T *)Ap,
+                                       (PcgState *)state, part1, (int64_t)g1, part2);
+      if (!fuse)
+        k_pcg_update<T><<<g, kRB, 0, s>>>(n, (const T *)z, (T *)p, (T *)p1, (PcgState *)state,
+                                          part2);
+    });
+    ESR_LAUNCHED();
+  }
+#undef ESR_STEP1_SELL
   if (count > 0 && fuse) {
     ESR_FLOAT_DISPATCH(dtype, (k_pcg_settle<T><<<g, kRB, 0, s>>>(n, (T *)p, (const T *)p1,
                                                                  (PcgState *)state, part2)));

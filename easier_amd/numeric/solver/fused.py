@@ -41,6 +41,11 @@ nat.register_signatures({
     'esr_pcg_init': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_dbl, _c_dbl, _c_i64, _c_vp]),
     'esr_pcg_steps': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i64, _c_vp]),
     'esr_pcg_read': (_c_i32, [_c_vp, _P_dbl, _P_i64, _c_vp]),
+    'esr_sell_plan': (_c_i32, [_c_i64] + [_c_vp] * 4 + [_P_i64, ctypes.POINTER(_c_i32), _c_vp]),
+    'esr_sell_fill': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 4 + [_c_i32] + [_c_vp] * 4),
+    'esr_sell_values': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 5),
+    'esr_pcg_steps_sell': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 3 + [_c_i32] + [_c_vp] * 10
+                           + [_c_i64, _c_vp]),
     'esr_gmres_workspace': (_c_i32, [_c_i64, _c_i64, _P_i64, _P_i64]),
     'esr_gmres_begin': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_gmres_residual': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 9 + [_c_vp]),
@@ -111,6 +116,60 @@ class LinsysCSR:
         return None
 
 
+class LinsysSELL:
+    """The operator of a LinsysCSR in sliced ELLPACK with slices of one
+    wavefront (SELL-64, csrc/esr_krylov.h): soff (slice offsets), cnt (row
+    lengths, uint8), colv (int16 offsets from the row when every one fits,
+    else int32 indices), val (values in `values`' dtype; the CSR's Ae by
+    default). The structure is fixed by the compiled Selector and Reducer;
+    refresh() copies new operator values in (a solve calls it first, so
+    values changed in place between solves are seen). Raises EsrNativeError
+    when a row has more than 255 entries (the CSR path is then used)."""
+
+    def __init__(self, csr: 'LinsysCSR', values: Optional[torch.Tensor] = None):
+        vals = csr.Ae if values is None else values
+        n = csr.n
+        dev = csr.Av.device
+        self.n, self.rowptr, self.dt = n, csr.rowptr, nat.dtype_code(vals.dtype)
+        self.soff = torch.empty((n + 63) // 64 + 1, dtype=torch.int64, device=dev)
+        flags = torch.empty(2, dtype=torch.int32, device=dev)
+        slots, idx16 = ctypes.c_int64(), ctypes.c_int32()
+        nat.call('esr_sell_plan', n, csr.rowptr.data_ptr(), csr.col.data_ptr(),
+                 self.soff.data_ptr(), flags.data_ptr(), ctypes.byref(slots),
+                 ctypes.byref(idx16), nat.stream_ptr())
+        self.idx16 = int(idx16.value)
+        self.slots = int(slots.value)
+        m = max(self.slots, 1)
+        self.cnt = torch.empty(n, dtype=torch.uint8, device=dev)
+        self.colv = torch.empty(m, dtype=torch.int16 if self.idx16 else torch.int32,
+                                device=dev)
+        self.val = torch.empty(m, dtype=vals.dtype, device=dev)
+        nat.call('esr_sell_fill', self.dt, n, csr.rowptr.data_ptr(), csr.col.data_ptr(),
+                 vals.data_ptr(), self.soff.data_ptr(), self.idx16, self.cnt.data_ptr(),
+                 self.colv.data_ptr(), self.val.data_ptr(), nat.stream_ptr())
+
+    def refresh(self, values: torch.Tensor):
+        if nat.dtype_code(values.dtype) != self.dt:
+            raise TypeError('SELL values keep their dtype')
+        nat.call('esr_sell_values', self.dt, self.n, self.rowptr.data_ptr(),
+                 values.data_ptr(), self.soff.data_ptr(), self.val.data_ptr(),
+                 nat.stream_ptr())
+
+    def arrays(self):
+        return [self.soff.data_ptr(), self.cnt.data_ptr(), self.colv.data_ptr(),
+                self.idx16, self.val.data_ptr()]
+
+
+def make_sell(csr: 'LinsysCSR', values: Optional[torch.Tensor] = None):
+    """LinsysSELL of the operator, or None when the SELL form does not apply
+    (a row of more than 255 entries)."""
+    try:
+        return LinsysSELL(csr, values)
+    except nat.EsrNativeError as e:
+        logger.info(f'Linsys kept in CSR form ({e})')
+        return None
+
+
 def _diag_ok(d, x: esr.Tensor, prog) -> bool:
     """A Jacobi diagonal the fused kernels take: a partition vector in x's
     space and dtype."""
@@ -165,6 +224,7 @@ class _Graph:
 
 class FusedCG:
     CHUNK = 32
+    use_sell = True      # the product on the SELL-64 form (identical results)
 
     def __init__(self, cg, prog, op=None, diag=None):
         self.cg = cg
@@ -179,6 +239,9 @@ class FusedCG:
         self.ws = torch.empty(wb.value, dtype=torch.uint8, device=dev)
         self.Ap = torch.empty(n, dtype=self.op.dtype, device=dev)
         self.dt = nat.dtype_code(self.op.dtype)
+        # the SELL form is built on the first solve (set-up needs the GPU)
+        self.want_sell = self.use_sell
+        self.sell = None
         self.graph = None
         self.graph_chunk = 0
 
@@ -190,10 +253,15 @@ class FusedCG:
 
     def steps(self, count, stream):
         c = self.cg
-        nat.call('esr_pcg_steps', self.dt, self.op.n, *self._arrays(),
-                 c.x.data.data_ptr(), c.r.data.data_ptr(), c.z.data.data_ptr(),
-                 c.p.data.data_ptr(), self.Ap.data_ptr(), self.state.data_ptr(),
-                 self.ws.data_ptr(), count, stream)
+        vecs = [c.x.data.data_ptr(), c.r.data.data_ptr(), c.z.data.data_ptr(),
+                c.p.data.data_ptr(), self.Ap.data_ptr(), self.state.data_ptr(),
+                self.ws.data_ptr(), count, stream]
+        if self.sell is not None:
+            nat.call('esr_pcg_steps_sell', self.dt, self.op.n, *self.sell.arrays(),
+                     self.op.Av.data_ptr(),
+                     None if self.diag is None else self.diag.data_ptr(), *vecs)
+        else:
+            nat.call('esr_pcg_steps', self.dt, self.op.n, *self._arrays(), *vecs)
 
     def read(self):
         sc = (ctypes.c_double * 6)()
@@ -207,6 +275,12 @@ class FusedCG:
         chunk graph is captured on first use."""
         c = self.cg
         self.bound.check()
+        if self.want_sell:
+            self.want_sell = False
+            self.sell = make_sell(self.op)
+            self.graph = None
+        elif self.sell is not None:
+            self.sell.refresh(self.op.Ae)
         nat.call('esr_pcg_init', self.dt, self.op.n, *self._arrays(),
                  c.b.data.data_ptr(), c.x.data.data_ptr(), c.r.data.data_ptr(),
                  c.z.data.data_ptr(), c.p.data.data_ptr(), self.state.data_ptr(),

@@ -24,7 +24,7 @@ from ...core import module as esr
 from ...core import ops
 from ...core.fuser import BoundTensors
 from ...core.utils import logger
-from .fused import LinsysCSR, _Graph, _precond_diag
+from .fused import FusedCG, LinsysCSR, _Graph, _precond_diag, make_sell
 
 _c_i32, _c_i64, _c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
 nat.register_signatures({
@@ -86,6 +86,8 @@ class MixedPrecisionCG(esr.Module):
         self.easier_state = s = dict(op=op, n=n, diag64=diag)
         s['bound'] = BoundTensors(op.sources + [self.b, self.x])
         s['Ae32'], s['Av32'], s['diag32'] = f32(op.Ae), f32(op.Av), f32(diag)
+        s['want_sell'] = FusedCG.use_sell      # built on the first solve
+        s['sell'] = None
         for k in ('r32', 'd32', 'ri', 'z', 'p', 'Ap'):
             s[k] = torch.zeros(n, dtype=torch.float32, device=dev)
         sb, wb = ctypes.c_int64(), ctypes.c_int64()
@@ -106,10 +108,31 @@ class MixedPrecisionCG(esr.Module):
 
     def _inner_steps(self, count, sp):
         s = self.easier_state
-        nat.call('esr_pcg_steps', nat.F32, s['n'], *self._inner_arrays(),
-                 s['d32'].data_ptr(), s['ri'].data_ptr(), s['z'].data_ptr(),
-                 s['p'].data_ptr(), s['Ap'].data_ptr(), s['pcg_state'].data_ptr(),
-                 s['pcg_ws'].data_ptr(), count, sp)
+        vecs = [s['d32'].data_ptr(), s['ri'].data_ptr(), s['z'].data_ptr(),
+                s['p'].data_ptr(), s['Ap'].data_ptr(), s['pcg_state'].data_ptr(),
+                s['pcg_ws'].data_ptr(), count, sp]
+        if s['sell'] is not None:
+            nat.call('esr_pcg_steps_sell', nat.F32, s['n'], *s['sell'].arrays(),
+                     s['Av32'].data_ptr(),
+                     None if s['diag32'] is None else s['diag32'].data_ptr(), *vecs)
+        else:
+            nat.call('esr_pcg_steps', nat.F32, s['n'], *self._inner_arrays(), *vecs)
+
+    def _refresh_operator(self):
+        """The float32 copies of the operator (and the Jacobi diagonal) from
+        the float64 tensors as they are now, so values changed in place
+        between solves are seen."""
+        s = self.easier_state
+        op = s['op']
+        for src, dst in ((op.Ae, s['Ae32']), (op.Av, s['Av32']), (s['diag64'], s['diag32'])):
+            if src is not None:
+                ops.ewise('copy', src, out=dst)
+        if s['want_sell']:
+            s['want_sell'] = False
+            s['sell'] = make_sell(op, s['Ae32'])
+            s['graph'] = None
+        elif s['sell'] is not None:
+            s['sell'].refresh(s['Ae32'])
 
     def _residual(self, first):
         s = self.easier_state
@@ -145,6 +168,7 @@ class MixedPrecisionCG(esr.Module):
             raise RuntimeError('run esr.compile() first')
         s = self.easier_state
         s['bound'].check()
+        self._refresh_operator()
         if s['graph'] is None or s.get('chunk') != chunk:
             s['graph'] = _Graph(lambda sp: self._inner_steps(chunk, sp))
             s['chunk'] = chunk

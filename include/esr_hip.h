@@ -294,6 +294,38 @@ int esr_pcg_rearm(int dtype, int64_t n, const void *diag, const void *r, void *z
 int esr_pcg_read(const void *state, double *scalars, int64_t *ints,
                  void *stream);
 
+/* ---- a3: the Linsys operator in sliced ELLPACK (SELL-64) ------------------
+ * The same operator as the CSR above (linsys.py:23-24: the Reducer's rows,
+ * the Selector's index, Ae), stored slice by slice: rows 64s..64s+63 form
+ * slice s, its width w_s the longest of those rows, entry k of row 64s+l at
+ * soff[s] + 64k + l; cnt[r] (uint8) the row's length; padded entries hold
+ * value 0. Column indices are int16 offsets from the row when every one fits
+ * (idx16 = 1), else int32 indices.
+ * esr_sell_plan: soff (device, ceil(n/64)+1 int64) from rowptr/col; flags
+ * (device, 2 int32) is scratch; returns the total slot count and idx16 on the
+ * host (synchronises the stream); ESR_ERR_UNSUPPORTED for a row of more than
+ * 255 entries. esr_sell_fill: cnt (n bytes), colv (slots int16 or int32),
+ * val (slots values of dtype) from the CSR. esr_sell_values: val again from
+ * new Ae values (same structure). esr_pcg_steps_sell: esr_pcg_steps with the
+ * product on the SELL form (identical results). No reference counterpart:
+ * the reference applies the operator as torch index_select + scatter_reduce
+ * (core/module.py:364-371, 441-461).                                        */
+int esr_sell_plan(int64_t n, const int32_t *rowptr, const int32_t *col,
+                  int64_t *soff, int32_t *flags, int64_t *slots, int32_t *idx16,
+                  void *stream);
+int esr_sell_fill(int dtype, int64_t n, const int32_t *rowptr,
+                  const int32_t *col, const void *Ae, const int64_t *soff,
+                  int32_t idx16, uint8_t *cnt, void *colv, void *val,
+                  void *stream);
+int esr_sell_values(int dtype, int64_t n, const int32_t *rowptr,
+                    const void *Ae, const int64_t *soff, void *val,
+                    void *stream);
+int esr_pcg_steps_sell(int dtype, int64_t n, const int64_t *soff,
+                       const uint8_t *cnt, const void *colv, int32_t idx16,
+                       const void *val, const void *Av, const void *diag,
+                       void *x, void *r, void *z, void *p, void *Ap,
+                       void *state, void *ws, int64_t count, void *stream);
+
 /* ---- a6: fused GMRES(m) Arnoldi cycle ---------------------------------------
  * easier/numeric/solver/gmres.py:12-266 (m <= 32). esr_gmres_begin: ||b||;
  * esr_gmres_residual: r = b - A x, ||r|| (UpdateRnorm); esr_gmres_cycle:
