@@ -169,7 +169,10 @@ def pcg_iteration_roofline(cg, n, nnz, T, chunks=3):
     torch.cuda.synchronize()
     it = a.elapsed_time(b) / 1e3 / (chunks * f.graph_chunk)
     bytes_ = nnz * (4 + T) + (n + 1) * 4 + 14 * n * T
-    return dict(bound='hbm', kernel='Jacobi-PCG iteration (k_pcg_step1/2/3 of one step)',
+    f = cg.easier_fused
+    k1 = 'k_pcg_step1_sell' if getattr(f, 'sell', None) is not None else 'k_pcg_step1'
+    return dict(bound='hbm', kernel=f'Jacobi-PCG iteration ({k1}, k_pcg_step2, k_pcg_update '
+                                    'of one step)',
                 achieved=bytes_ / it / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                 frac=bytes_ / it / HBM_PEAK, traffic=None, bytes_per_launch=bytes_,
                 avg_iteration_s=it)
