@@ -553,6 +553,57 @@ __global__ __launch_bounds__(kRB) void k_gm_dots(int64_t n, int j, const T *__re
   gs_store_partials<JB>(acc, j + 1, part_out);
 }
 
+// k_gm_spmv and k_gm_dots as one kernel (systems of at most kFuseUpdateMax
+// rows, where the Arnoldi step is launch-bound): on the dots kernel's grid
+// and rows, each thread forms V_j[r] and w[r] = A(M(V_j))[r], then adds
+// V_i[r] w[r] (i < j from memory, i >= j its own V_j[r]) to the partials of
+// h_i -- the values, products and summation order of the two kernels, so
+// the same bits.
+template <typename T, int JB>
+__global__ __launch_bounds__(kRB) void k_gm_spmv_dots(
+    int64_t n, int j, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ diag,
+    const T *__restrict__ src, T *__restrict__ V, T *__restrict__ w, GmresState *st,
+    const double *__restrict__ part_in, int64_t g_in, double *__restrict__ part_out) {
+  if (__hip_atomic_load(&st->broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  T s;
+  if (j == 0) {
+    s = (T)st->rnorm;
+  } else {
+    double tot[1];
+    sum_partials<1>(part_in, g_in, 0, g_in, tot);
+    const double hn = (double)(T)cr_sqrt(tot[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->H[j * kGmresMaxRestart + (j - 1)] = hn;
+      st->jend = j - 1;
+      if (hn < 1e-15)
+        __hip_atomic_store(&st->broken, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (hn < 1e-15) return;
+    s = (T)hn;
+  }
+  auto mv = [&](int64_t c) {
+    const T v = src[c] / s;
+    return diag ? v / diag[c] : v;
+  };
+  T *__restrict__ Vj = V + (int64_t)j * n;
+  double acc[JB];
+#pragma unroll
+  for (int i = 0; i < JB; ++i) acc[i] = 0.0;
+  ESR_GRID_ROWS(r, n) {
+    const T vj = src[r] / s;
+    Vj[r] = vj;
+    const T wr = linsys_row<T>(r, rowptr, col, Ae, Av, mv);
+    w[r] = wr;
+#pragma unroll
+    for (int i = 0; i < JB; ++i) {
+      const T vi = i < j ? V[(int64_t)i * n + r] : vj;
+      acc[i] += (double)(vi * wr);
+    }
+  }
+  gs_store_partials<JB>(acc, j + 1, part_out);
+}
+
 template <typename T, int JB, bool FIRST>
 __global__ __launch_bounds__(kRB) void k_gm_cgs(int64_t n, int j, const T *__restrict__ V,
                                                T *__restrict__ w, GmresState *st,
@@ -936,6 +987,8 @@ ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t
               "esr_gmres_cycle: null argument");
   hipStream_t s = as_stream(stream);
   const unsigned g = red_grid(n), gg = gs_grid(n);
+  // launch-bound systems: the product and the first dots in one kernel
+  const bool fuse = n <= kFuseUpdateMax;
   double *reg[2] = {(double *)ws, (double *)ws + (int64_t)kGmresMaxRestart * gg};
   ESR_FLOAT_DISPATCH(dtype, {
     T *Vt = (T *)V;
@@ -945,12 +998,18 @@ ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t
       const T *src = j == 0 ? (const T *)r : w[(j - 1) & 1];
       T *wj = w[j & 1];
       // the norm partials of w_{j-1} are in reg[0] (written by k_gm_cgs<0>)
-      k_gm_spmv<T><<<g, kRB, 0, s>>>(n, j, rowptr, col, (const T *)Ae, (const T *)Av,
-                                     (const T *)diag, src, Vt + (int64_t)j * n, wj, st,
-                                     reg[0], (int64_t)gg);
+      if (!fuse)
+        k_gm_spmv<T><<<g, kRB, 0, s>>>(n, j, rowptr, col, (const T *)Ae, (const T *)Av,
+                                       (const T *)diag, src, Vt + (int64_t)j * n, wj, st,
+                                       reg[0], (int64_t)gg);
       auto orth = [&](auto jb) {
         constexpr int JB = decltype(jb)::value;
-        k_gm_dots<T, JB><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[1]);
+        if (fuse)
+          k_gm_spmv_dots<T, JB><<<gg, kRB, 0, s>>>(n, j, rowptr, col, (const T *)Ae,
+                                                   (const T *)Av, (const T *)diag, src, Vt, wj,
+                                                   st, reg[0], (int64_t)gg, reg[1]);
+        else
+          k_gm_dots<T, JB><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[1]);
         k_gm_cgs<T, JB, true><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[1], reg[0]);
         k_gm_cgs<T, JB, false><<<gg, kRB, 0, s>>>(n, j, Vt, wj, st, reg[0], reg[1]);
       };
