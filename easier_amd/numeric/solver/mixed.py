@@ -85,7 +85,10 @@ class MixedPrecisionCG(esr.Module):
         f32 = lambda t: None if t is None else t.to(torch.float32)
         self.easier_state = s = dict(op=op, n=n, diag64=diag)
         s['bound'] = BoundTensors(op.sources + [self.b, self.x])
-        s['Ae32'], s['Av32'], s['diag32'] = f32(op.Ae), f32(op.Av), f32(diag)
+        s['Ae32'], s['Av32'] = f32(op.Ae), f32(op.Av)
+        # M(r) = r / A.Av (the usual Jacobi form): one float32 copy serves both
+        same = diag is not None and diag.data_ptr() == op.Av.data_ptr()
+        s['diag32'] = s['Av32'] if same else f32(diag)
         s['want_sell'] = FusedCG.use_sell      # built on the first solve
         s['sell'] = None
         for k in ('r32', 'd32', 'ri', 'z', 'p', 'Ap'):
@@ -125,7 +128,7 @@ class MixedPrecisionCG(esr.Module):
         s = self.easier_state
         op = s['op']
         for src, dst in ((op.Ae, s['Ae32']), (op.Av, s['Av32']), (s['diag64'], s['diag32'])):
-            if src is not None:
+            if src is not None and not (dst is s['Av32'] and src is not op.Av):
                 ops.ewise('copy', src, out=dst)
         if s['want_sell']:
             s['want_sell'] = False
