@@ -20,32 +20,38 @@
 
 namespace {
 
-struct EdgeRec {
-  int64_t a, b;  // a < b
-  int64_t t;     // triangle
-};
-
+// The distinct undirected triangle edges (a < b) in (a, b) order, with the
+// number of triangles sharing each: a counting sort by a (one bucket per
+// vertex), then each bucket -- a vertex's few larger neighbours -- sorted.
 bool build_edges(int64_t np_, int64_t nc, const int64_t *cells, std::vector<int64_t> &ua,
                  std::vector<int64_t> &ub, std::vector<int64_t> &count) {
-  std::vector<EdgeRec> recs;
-  recs.reserve(3 * (size_t)nc);
+  std::vector<int64_t> head((size_t)np_ + 1, 0);
   for (int64_t t = 0; t < nc; ++t) {
     for (int k = 0; k < 3; ++k) {
-      int64_t u = cells[3 * t + k], v = cells[3 * t + (k + 1) % 3];
+      const int64_t u = cells[3 * t + k], v = cells[3 * t + (k + 1) % 3];
       if (u < 0 || u >= np_ || v < 0 || v >= np_ || u == v) return false;
-      recs.push_back({std::min(u, v), std::max(u, v), t});
+      head[(size_t)std::min(u, v) + 1]++;
     }
   }
-  std::stable_sort(recs.begin(), recs.end(), [](const EdgeRec &x, const EdgeRec &y) {
-    return x.a != y.a ? x.a < y.a : x.b < y.b;
-  });
-  for (size_t i = 0; i < recs.size();) {
-    size_t j = i;
-    while (j < recs.size() && recs[j].a == recs[i].a && recs[j].b == recs[i].b) ++j;
-    ua.push_back(recs[i].a);
-    ub.push_back(recs[i].b);
-    count.push_back((int64_t)(j - i));
-    i = j;
+  for (int64_t a = 0; a < np_; ++a) head[a + 1] += head[a];
+  std::vector<int64_t> pos(head.begin(), head.end() - 1), nb((size_t)head[np_]);
+  for (int64_t t = 0; t < nc; ++t) {
+    for (int k = 0; k < 3; ++k) {
+      const int64_t u = cells[3 * t + k], v = cells[3 * t + (k + 1) % 3];
+      nb[(size_t)pos[std::min(u, v)]++] = std::max(u, v);
+    }
+  }
+  for (int64_t a = 0; a < np_; ++a) {
+    int64_t *lo = nb.data() + head[a], *hi = nb.data() + head[a + 1];
+    std::sort(lo, hi);
+    for (int64_t *i = lo; i < hi;) {
+      int64_t *j = i;
+      while (j < hi && *j == *i) ++j;
+      ua.push_back(a);
+      ub.push_back(*i);
+      count.push_back((int64_t)(j - i));
+      i = j;
+    }
   }
   return true;
 }

@@ -9,6 +9,7 @@ of the tutorial's finite-volume one.
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 import torch
@@ -43,6 +44,41 @@ def p1_system(mesh, kappa_cell=None, f=None):
              col.ctypes.data, Ae.ctypes.data, Av.ctypes.data, b.ctypes.data,
              bnd.ctypes.data)
     return dict(row=row, col=col, Ae=Ae, Av=Av, b=b, boundary=bnd.astype(bool))
+
+
+def load_vectors(mesh, fs, boundary):
+    """b = M f for each row f of fs (k x vertices): the consistent P1 mass
+    matrix (|T|/12 (1 + [i == j]) per triangle), zero on boundary vertices --
+    the load vector of p1_system, contributions added in the same triangle
+    order (so the same bits) -- for many right-hand sides of one mesh without
+    re-assembling the operator."""
+    pts = np.asarray(mesh['points'], np.float64)
+    c = np.asarray(mesh['cells'], np.int64)
+    xa, ya = pts[c[:, 0], 0], pts[c[:, 0], 1]
+    xb, yb = pts[c[:, 1], 0], pts[c[:, 1], 1]
+    xc, yc = pts[c[:, 2], 0], pts[c[:, 2], 1]
+    cross = (xb - xa) * (yc - ya) - (xc - xa) * (yb - ya)
+    m = (0.5 * np.abs(cross)) / 12.0
+    bnd = np.asarray(boundary, bool)
+    out = np.empty((len(fs), pts.shape[0]))
+    flat = c.ravel()
+
+    def one(k):
+        f = np.asarray(fs[k], np.float64)
+        fv = f[c]                                   # (triangles, 3)
+        contrib = np.empty_like(fv)
+        for i in range(3):
+            contrib[:, i] = m * (2.0 * fv[:, i] + fv[:, (i + 1) % 3] + fv[:, (i + 2) % 3])
+        out[k] = np.bincount(flat, weights=contrib.ravel(), minlength=pts.shape[0])
+        out[k, bnd] = 0.0
+
+    if len(fs) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(min(len(fs), os.cpu_count() or 1, 16)) as ex:
+            list(ex.map(one, range(len(fs))))
+    elif fs:
+        one(0)
+    return out
 
 
 def gaussian_random_field(n, seed, ell=0.05, variance=1.0):

@@ -84,3 +84,21 @@ def test_gaussian_random_field_statistics():
     c = (zs[:, :, :-lag] * zs[:, :, lag:]).mean()
     assert abs(c - math.exp(-0.5 * (lag / n / ell) ** 2)) < 0.1
     assert np.array_equal(wp.gaussian_random_field(n, 3), wp.gaussian_random_field(n, 3))
+
+
+def test_load_vectors_equal_the_assemblers_bits():
+    """workloads.poisson.load_vectors (many right-hand sides of one mesh in
+    one pass) gives the load vector esr_p1_assemble computes, bit for bit, on
+    a structured and a jittered mesh, boundary rows zero."""
+    import numpy as np
+    from easier_amd.workloads import mesh as wmesh
+    from easier_amd.workloads import poisson as wp
+    for m in (wmesh.tri2(9), wmesh.tri2(33, jitter=0.2, seed=5)):
+        nv = m['points'].shape[0]
+        fs = [wp.rhs_uniform(nv, k) for k in range(4)]
+        base = wp.p1_system(m, None, None)
+        B = wp.load_vectors(m, fs, base['boundary'])
+        for k in range(4):
+            want = wp.p1_system(m, None, fs[k])['b']
+            assert np.array_equal(B[k], want)
+        assert not B[:, base['boundary']].any()
