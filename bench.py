@@ -366,8 +366,10 @@ def poisson_gmres_config1(skip_cpu):
     return out
 
 
-def swe_side(n, seed, dtype, freeze, steps=100, warmup=5):
-    """The SWE step in another precision or compile mode (cells/s)."""
+def swe_side(n, seed, dtype, freeze, steps=100, warmup=5, with_primitives=False):
+    """The SWE step in another precision or compile mode (cells/s); with
+    with_primitives, also the plain Selector / Reducer / aggregator kernels
+    on this compile's mesh order (RCM when not frozen)."""
     import torch
     mesh, geo, eqn, meta = build_swe(n, dtype, seed, freeze=freeze,
                                      reorder='blocks' if freeze else 'rcm')
@@ -380,9 +382,13 @@ def swe_side(n, seed, dtype, freeze, steps=100, warmup=5):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     nc = mesh['cells'].shape[0]
-    return dict(workload=f'shallow-water RK4 {n}x{n}, {dtype}, freeze={freeze}', steps=steps,
-                ms_per_step=1e3 * wall / steps, cells_per_s=nc * steps / wall,
-                state_finite=bool(torch.isfinite(eqn.h.collect()).all()))
+    out = dict(workload=f'shallow-water RK4 {n}x{n}, {dtype}, freeze={freeze}', steps=steps,
+               ms_per_step=1e3 * wall / steps, cells_per_s=nc * steps / wall,
+               state_finite=bool(torch.isfinite(eqn.h.collect()).all()))
+    if with_primitives:
+        out['primitives'] = primitives(eqn)
+        out['primitives']['mesh_order'] = 'rcm' if not freeze else 'blocks'
+    return out
 
 
 def committed_traffic(n, dtype, kernel, tag=''):
@@ -735,6 +741,7 @@ def main():
         log(f'dominant kernel esr_fused_{k}: {ktimes[k] * 1e6:.1f} us, '
             f'{achieved / 1e9:.0f} GB/s ({achieved / HBM_PEAK:.1%} of 8 TB/s)')
     result['primitives'] = primitives(eqn, with_cpu=not args.skip_cpu)
+    result['primitives']['mesh_order'] = reorder
     if not args.skip_cpu:
         log(f'CPU baseline: {args.cpu_steps} oracle steps on the host')
         result['cpu_baseline'] = cpu_baseline_swe(mesh, geo, meta, dtype, args.cpu_steps)
@@ -752,7 +759,7 @@ def main():
             log('SWE float64 without freeze')
             torch.cuda.empty_cache()
             result['swe_fp64_unfrozen'] = swe_side(args.n, args.seed, torch.float64, False,
-                                                   steps=40)
+                                                   steps=40, with_primitives=True)
     extra = {c.strip() for c in args.configs.split(',') if c.strip()}
     if '3' in extra:
         result['batched_config3'] = batched_config3()

@@ -162,10 +162,29 @@ __global__ __launch_bounds__(kFB) void k_gnn_encode(int64_t n, const double *__r
   if (ln.ok) *reinterpret_cast<bf16x8 *>(Q0 + ln.v * H + 8 * ln.g) = to_bf16x8(qv);
 }
 
-template <bool LAST>
+// Edges of a vertex, in CSR order. CSR: rowptr / col / efeat (3 floats per
+// edge). Slab (SLAB = true): the edges of the wavefront's 16 vertices
+// stored slot-major -- edge k of vertex 16 q + j at slot soff[q] + 16 k + j,
+// group width (soff[q + 1] - soff[q]) / 16 = its largest degree, ecnt[v] the
+// degree, padded slots a valid column and zero features -- so the column and
+// feature loads of one slot are one contiguous access per wavefront instead
+// of 16 rows' records spread over many cache lines, and they do not wait on
+// a row-pointer load. The same edges in the same order either way.
+struct EdgeCsr {
+  const int32_t *__restrict__ rowptr;
+  const int32_t *__restrict__ col;
+  const float *__restrict__ feat;
+};
+struct EdgeSlab {
+  const int64_t *__restrict__ soff;
+  const uint8_t *__restrict__ cnt;
+  const int32_t *__restrict__ col;
+  const float *__restrict__ feat;
+};
+
+template <bool LAST, typename E>
 __global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gnn_layer(
-    int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const float *__restrict__ efeat, const __bf16 *__restrict__ h_in,
+    int64_t n, const E edges, const __bf16 *__restrict__ h_in,
     const __bf16 *__restrict__ Q_in, const float *__restrict__ lw,
     const __bf16 *__restrict__ lp, const __bf16 *__restrict__ WqT_next,
     __bf16 *__restrict__ h_out, __bf16 *__restrict__ Q_out, float *__restrict__ a_save,
@@ -173,30 +192,51 @@ __global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const double *__restrict__ d, const double *__restrict__ rnorm, double *__restrict__ z) {
   const Lane ln = lane_of(n);
   const int g = ln.g, j16 = threadIdx.x & 15;
-  // the row's extent and h first: the edge loads wait on rowptr only
+  // edge k of this vertex: position pos(k) in the edge arrays for k < deg;
+  // for CSR the row's extent comes first (the edge loads wait on rowptr),
+  // for the slab the group's base and width (wave-uniform) and the degree
+  int64_t e0 = 0;
+  int deg = 0, width = 0;
   bf16x8 hb = bf16x8{};
-  int32_t e0 = 0, e1 = 0;
-  if (ln.ok) {
-    e0 = rowptr[ln.v];
-    e1 = rowptr[ln.v + 1];
-    hb = *reinterpret_cast<const bf16x8 *>(h_in + ln.v * H + 8 * g);
+  if constexpr (std::is_same<E, EdgeSlab>::value) {
+    const int64_t q = ln.v >> 4;
+    const int64_t b0 = edges.soff[q];
+    width = (int)((edges.soff[q + 1] - b0) >> 4);
+    e0 = b0 + j16;
+    if (ln.ok) deg = edges.cnt[ln.v];
+  } else {
+    if (ln.ok) {
+      e0 = edges.rowptr[ln.v];
+      deg = edges.rowptr[ln.v + 1] - (int32_t)e0;
+    }
   }
-  // The first kSlots edges: positions clamped to the row, so every load is
-  // valid and all are in flight together. Lane (g, vertex) takes component
-  // g of the edge's features (g = 3: the constant 1 that carries bm).
+  if (ln.ok) hb = *reinterpret_cast<const bf16x8 *>(h_in + ln.v * H + 8 * g);
+  // position of edge k (k clamped to the slots that exist, so every load is
+  // valid): CSR e0 + min(k, deg - 1), slab e0 + 16 min(k, width - 1)
+  auto pos = [&](int k) -> int64_t {
+    if constexpr (std::is_same<E, EdgeSlab>::value)
+      return e0 + 16 * (int64_t)(k < width ? k : width - 1);
+    else
+      return e0 + (k < deg ? k : deg - 1);
+  };
+  const bool any = std::is_same<E, EdgeSlab>::value ? width > 0 : deg > 0;
+  // The first kSlots edges, all loads in flight together. Lane (g, vertex)
+  // takes component g of the edge's features (g = 3: the constant 1 that
+  // carries bm).
   constexpr int kSlots = 6;
   bf16x8 qs[kSlots];
   float ef[kSlots];
-  if (e1 > e0) {
-    int32_t es[kSlots], js[kSlots];
+  if (any) {
+    int64_t es[kSlots];
+    int32_t js[kSlots];
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) es[k] = e0 + k < e1 ? e0 + k : e1 - 1;
+    for (int k = 0; k < kSlots; ++k) es[k] = pos(k);
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) js[k] = col[es[k]];
+    for (int k = 0; k < kSlots; ++k) js[k] = edges.col[es[k]];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) {
       qs[k] = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)js[k] * H + 8 * g);
-      ef[k] = g < 3 ? efeat[3 * (int64_t)es[k] + g] : 1.0f;
+      ef[k] = g < 3 ? edges.feat[3 * es[k] + g] : 1.0f;
     }
   } else {
 #pragma unroll
@@ -233,24 +273,25 @@ __global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   for (int k = 0; k < kSlots; ++k) {
     const floatx4 o0 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[0], ef[k], Pc[0], 0, 0, 0);
     const floatx4 o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[1], ef[k], Pc[1], 0, 0, 0);
-    if (e0 + k < e1) fold(o0, o1, qs[k]);
+    if (k < deg) fold(o0, o1, qs[k]);
   }
   // rows longer than kSlots: a trip count uniform over the wavefront (the
   // matrix-core instructions need every lane), loads and fold per row
-  int rest = e1 - e0 - kSlots;
+  int rest = deg - kSlots;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const int o = __shfl_xor(rest, off, 64);
     rest = o > rest ? o : rest;
   }
   for (int t = 0; t < rest; ++t) {
-    const int32_t e = e0 + kSlots + t;
-    const bool act = e < e1;
+    const bool act = kSlots + t < deg;
     float eft = 0.0f;
     bf16x8 qt = bf16x8{};
     if (act) {
-      eft = g < 3 ? efeat[3 * (int64_t)e + g] : 1.0f;
-      qt = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)col[e] * H + 8 * g);
+      const int64_t e = std::is_same<E, EdgeSlab>::value ? e0 + 16 * (int64_t)(kSlots + t)
+                                                          : e0 + kSlots + t;
+      eft = g < 3 ? edges.feat[3 * e + g] : 1.0f;
+      qt = *reinterpret_cast<const bf16x8 *>(Q_in + (int64_t)edges.col[e] * H + 8 * g);
     }
     const floatx4 o0 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[0], eft, Pc[0], 0, 0, 0);
     const floatx4 o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[1], eft, Pc[1], 0, 0, 0);
@@ -534,17 +575,11 @@ ESR_API int esr_gnn_pack(int L, const float *params, void *packed, void *stream)
 // z = M(r). h: (save ? L+1 : 2) buffers of n*H bf16; Q: (save ? L : 2)
 // buffers; a: L buffers of n*H float32 when save != 0 (else NULL).
 // rnorm: device float64 scalar ||r||.
-ESR_API int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr, const int32_t *col,
-                            const float *efeat, const float *vfeat, const double *r,
-                            const double *d, const double *rnorm, const float *params,
-                            const void *packed, void *h, void *Q, float *a, int save,
-                            double *z, void *stream) {
-  ESR_GUARD_BEGIN
-  ESR_REQUIRE(L >= 1 && n >= 1 && rowptr && col && efeat && vfeat && r && d && rnorm && params &&
-                  packed && h && Q && z,
-              "esr_gnn_forward: null argument");
-  ESR_REQUIRE(!save || a, "esr_gnn_forward: save needs the a buffers");
-  hipStream_t s = as_stream(stream);
+namespace {
+template <typename E>
+int gnn_forward(int L, int64_t n, const E &edges, const float *vfeat, const double *r,
+                const double *d, const double *rnorm, const float *params, const void *packed,
+                void *h, void *Q, float *a, int save, double *z, hipStream_t s) {
   const Off o{L};
   const __bf16 *pk = (const __bf16 *)packed;
   __bf16 *hb = (__bf16 *)h, *qb = (__bf16 *)Q;
@@ -557,17 +592,49 @@ ESR_API int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr, const int32
     const __bf16 *lp = pk + (int64_t)l * kPLayer;
     float *asv = save ? a + (int64_t)l * n * H : nullptr;
     if (l + 1 < L) {
-      k_gnn_layer<false><<<vgrid(n), kFB, 0, s>>>(
-          n, rowptr, col, efeat, hbuf(l), qbuf(l), lw, lp, pk + (int64_t)(l + 1) * kPLayer + kPWq,
+      k_gnn_layer<false, E><<<vgrid(n), kFB, 0, s>>>(
+          n, edges, hbuf(l), qbuf(l), lw, lp, pk + (int64_t)(l + 1) * kPLayer + kPWq,
           hbuf(l + 1), qbuf(l + 1), asv, nullptr, nullptr, nullptr, nullptr, nullptr);
     } else {
-      k_gnn_layer<true><<<vgrid(n), kFB, 0, s>>>(
-          n, rowptr, col, efeat, hbuf(l), qbuf(l), lw, lp, nullptr, save ? hbuf(l + 1) : nullptr,
-          nullptr, asv, params + o.w_out(), r, d, rnorm, z);
+      k_gnn_layer<true, E><<<vgrid(n), kFB, 0, s>>>(
+          n, edges, hbuf(l), qbuf(l), lw, lp, nullptr, save ? hbuf(l + 1) : nullptr, nullptr,
+          asv, params + o.w_out(), r, d, rnorm, z);
     }
     ESR_LAUNCHED();
   }
   return ESR_OK;
+}
+} // namespace
+
+ESR_API int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr, const int32_t *col,
+                            const float *efeat, const float *vfeat, const double *r,
+                            const double *d, const double *rnorm, const float *params,
+                            const void *packed, void *h, void *Q, float *a, int save,
+                            double *z, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(L >= 1 && n >= 1 && rowptr && col && efeat && vfeat && r && d && rnorm && params &&
+                  packed && h && Q && z,
+              "esr_gnn_forward: null argument");
+  ESR_REQUIRE(!save || a, "esr_gnn_forward: save needs the a buffers");
+  return gnn_forward(L, n, EdgeCsr{rowptr, col, efeat}, vfeat, r, d, rnorm, params, packed, h, Q,
+                     a, save, z, as_stream(stream));
+  ESR_GUARD_END
+}
+
+// The same network with the edges in the slab layout (EdgeSlab): soff has
+// ceil(n / 16) + 1 entries, ecol and efeat (3 floats) soff[last] slots.
+ESR_API int esr_gnn_forward_slab(int L, int64_t n, const int64_t *soff, const uint8_t *ecnt,
+                                 const int32_t *ecol, const float *efeat, const float *vfeat,
+                                 const double *r, const double *d, const double *rnorm,
+                                 const float *params, const void *packed, void *h, void *Q,
+                                 float *a, int save, double *z, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(L >= 1 && n >= 1 && soff && ecnt && ecol && efeat && vfeat && r && d && rnorm &&
+                  params && packed && h && Q && z,
+              "esr_gnn_forward_slab: null argument");
+  ESR_REQUIRE(!save || a, "esr_gnn_forward_slab: save needs the a buffers");
+  return gnn_forward(L, n, EdgeSlab{soff, ecnt, ecol, efeat}, vfeat, r, d, rnorm, params, packed,
+                     h, Q, a, save, z, as_stream(stream));
   ESR_GUARD_END
 }
 

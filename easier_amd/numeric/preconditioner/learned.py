@@ -29,6 +29,7 @@ nat.register_signatures({
     'esr_gnn_sizes': (_c_i32, [_c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
     'esr_gnn_pack': (_c_i32, [_c_i32, _c_vp, _c_vp, _c_vp]),
     'esr_gnn_forward': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i32, _c_vp, _c_vp]),
+    'esr_gnn_forward_slab': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 13 + [_c_i32, _c_vp, _c_vp]),
     'esr_gnn_backward': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 16 + [_c_i64, _c_vp]),
     'esr_gnn_adam': (_c_i32, [_c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_dbl,
                               _c_dbl, _c_i64, _c_dbl, _c_vp]),
@@ -89,13 +90,77 @@ def unflatten(flat: torch.Tensor, L) -> Dict[str, torch.Tensor]:
     return out
 
 
+class EdgeSlab:
+    """The graph's edges for the network's forward kernels, slot-major per
+    group of 16 vertices (the vertices of one wavefront): edge k of vertex
+    16 q + j at slot soff[q] + 16 k + j, group width the group's largest
+    degree; ecnt the degrees (uint8), ecol the neighbour of each slot and
+    feat its 3 features (padded slots: the slot's own vertex, zero
+    features). The same edges in the same order as the CSR (rowptr, col,
+    efeat) it is built from. Raises ValueError for a degree above 255."""
+
+    def __init__(self, rowptr, col, efeat, n, device):
+        rp = np.asarray(rowptr, np.int64)
+        col = np.asarray(col, np.int64)
+        efeat = np.asarray(efeat, np.float32).reshape(-1, 3)
+        deg = np.diff(rp)
+        if deg.size and deg.max() > 255:
+            raise ValueError('a vertex has more than 255 edges')
+        ng = (n + 15) // 16
+        dp = np.zeros(ng * 16, np.int64)
+        dp[:n] = deg
+        width = dp.reshape(ng, 16).max(axis=1)
+        soff = np.zeros(ng + 1, np.int64)
+        soff[1:] = np.cumsum(16 * width)
+        slots = int(soff[-1])
+        # the vertex each slot belongs to (padding rows past n: vertex n - 1)
+        base = np.repeat(soff[:-1], 16 * width)
+        vert = np.repeat(np.arange(ng, dtype=np.int64) * 16, 16 * width) + \
+            (np.arange(slots, dtype=np.int64) - base) % 16
+        ecol = np.minimum(vert, n - 1)
+        rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+        k = np.arange(rp[-1], dtype=np.int64) - rp[rows]
+        slot = soff[rows // 16] + 16 * k + rows % 16
+        ecol[slot] = col
+        feat = np.zeros((max(slots, 1), 3), np.float32)
+        feat[slot] = efeat
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        self.soff = t(soff)
+        self.ecnt = t(deg.astype(np.uint8))
+        self.ecol = t(np.concatenate([ecol, np.zeros(1 if slots == 0 else 0, np.int64)])
+                      .astype(np.int32))
+        self.feat = t(feat)
+        self.slots = slots
+
+    @staticmethod
+    def of(graph):
+        """EdgeSlab of a graph with n, rowptr, col, efeat (device tensors)."""
+        return EdgeSlab(graph.rowptr.cpu().numpy(), graph.col.cpu().numpy(),
+                        graph.efeat.cpu().numpy(), graph.n, graph.rowptr.device)
+
+
+def _forward_call(pre, g, r, rnorm, h, Q, a, save, z):
+    """esr_gnn_forward on the graph's slab when it has one, else its CSR."""
+    tail = [g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(), rnorm.data_ptr(),
+            pre.params.data_ptr(), pre.packed.data_ptr(), h.data_ptr(), Q.data_ptr(),
+            None if a is None else a.data_ptr(), save, z.data_ptr(), nat.stream_ptr()]
+    slab = getattr(g, 'slab', None)
+    if slab is not None:
+        nat.call('esr_gnn_forward_slab', pre.L, g.n, slab.soff.data_ptr(),
+                 slab.ecnt.data_ptr(), slab.ecol.data_ptr(), slab.feat.data_ptr(), *tail)
+    else:
+        nat.call('esr_gnn_forward', pre.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
+                 g.efeat.data_ptr(), *tail)
+
+
 class MeshGraph:
     """Static inputs of the network for one operator: the CSR (row-sorted
     directed vertex edges), the reverse edge of every edge, vertex features
     (log(d/mean d), log kappa) and edge features (a_ij/sqrt(d_i d_j), edge
     vector in grid units), and the diagonal d."""
 
-    def __init__(self, row, col, Av, Ae, points, logk_vertex, n_grid, device=None):
+    def __init__(self, row, col, Av, Ae, points, logk_vertex, n_grid, device=None,
+                 use_slab=True):
         dev = device or torch.device('cuda', torch.cuda.current_device())
         row = np.asarray(row, np.int64)
         col = np.asarray(col, np.int64)
@@ -125,6 +190,7 @@ class MeshGraph:
         self.efeat = torch.from_numpy(np.ascontiguousarray(efeat)).to(dev)
         self.d = torch.from_numpy(Av).to(dev)
         self.device = dev
+        self.slab = EdgeSlab(rowptr, col, efeat, n, dev) if use_slab else None
 
 
 class GnnBuffers:
@@ -197,12 +263,7 @@ class LearnedPreconditioner:
         nothing, so it can be captured in a HIP graph."""
         g = self.graph
         ops.aggregate('norm', r.reshape(-1, 1), out=self.rnorm, ws=self.norm_ws)
-        nat.call('esr_gnn_forward', self.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
-                 g.efeat.data_ptr(), g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(),
-                 self.rnorm.data_ptr(), self.params.data_ptr(), self.packed.data_ptr(),
-                 self.h.data_ptr(), self.Q.data_ptr(),
-                 None if self.a is None else self.a.data_ptr(), 1 if self.train else 0,
-                 z.data_ptr(), nat.stream_ptr())
+        _forward_call(self, g, r, self.rnorm, self.h, self.Q, self.a, 1 if self.train else 0, z)
         return z
 
     def forward_into(self, g, bufs: 'GnnBuffers', r: torch.Tensor, z: torch.Tensor):
@@ -210,11 +271,7 @@ class LearnedPreconditioner:
         d) with the inference buffers bufs -- several meshes at once, e.g. the
         levels of a multigrid hierarchy. Allocates nothing."""
         ops.aggregate('norm', r.reshape(-1, 1), out=bufs.rnorm, ws=bufs.norm_ws)
-        nat.call('esr_gnn_forward', self.L, g.n, g.rowptr.data_ptr(), g.col.data_ptr(),
-                 g.efeat.data_ptr(), g.vfeat.data_ptr(), r.data_ptr(), g.d.data_ptr(),
-                 bufs.rnorm.data_ptr(), self.params.data_ptr(), self.packed.data_ptr(),
-                 bufs.h.data_ptr(), bufs.Q.data_ptr(), None, 0, z.data_ptr(),
-                 nat.stream_ptr())
+        _forward_call(self, g, r, bufs.rnorm, bufs.h, bufs.Q, None, 0, z)
         return z
 
     def backward(self, r: torch.Tensor, dz: torch.Tensor) -> torch.Tensor:

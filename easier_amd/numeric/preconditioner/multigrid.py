@@ -86,6 +86,7 @@ class _Graph:
     def __init__(self, n, rowptr, col, efeat, vfeat, d):
         self.n, self.rowptr, self.col, self.efeat, self.vfeat, self.d = \
             n, rowptr, col, efeat, vfeat, d
+        self.slab = None
 
 
 class Level:
@@ -209,7 +210,7 @@ class MultigridPreconditioner:
         return self
 
     def _setup_learned(self, logk, graph0, fpos):
-        from .learned import GnnBuffers
+        from .learned import EdgeSlab, GnnBuffers
         if logk is None or graph0 is None:
             raise ValueError('a learned smoother needs logk and the fine-level MeshGraph')
         dev = self.levels[0].Av.device
@@ -236,6 +237,7 @@ class MultigridPreconditioner:
                          L.Ae.data_ptr(), L.Av.data_ptr(), dsum.data_ptr(), lkl.data_ptr(),
                          vfeat.data_ptr(), efeat.data_ptr(), nat.stream_ptr())
                 L.graph = _Graph(L.nv, L.rowptr, L.col, efeat, vfeat, L.Av)
+                L.graph.slab = EdgeSlab.of(L.graph)
             L.bufs = GnnBuffers(L.nv, dev)
             L.s = torch.zeros(L.nv, dtype=torch.float64, device=dev)
 

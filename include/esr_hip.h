@@ -425,6 +425,17 @@ int esr_gnn_forward(int L, int64_t n, const int32_t *rowptr,
                     const double *r, const double *d, const double *rnorm,
                     const float *params, const void *packed, void *h, void *Q,
                     float *a, int save, double *z, void *stream);
+/* esr_gnn_forward with the edges slot-major per group of 16 vertices (one
+ * wavefront's): edge k of vertex 16q+j at slot soff[q] + 16k + j (soff:
+ * ceil(n/16)+1 int64), ecnt the degrees (uint8), ecol / efeat (3 floats) per
+ * slot, padded slots with a valid column and zero features; the same
+ * results as esr_gnn_forward on the CSR they are built from.                */
+int esr_gnn_forward_slab(int L, int64_t n, const int64_t *soff,
+                         const uint8_t *ecnt, const int32_t *ecol,
+                         const float *efeat, const float *vfeat,
+                         const double *r, const double *d, const double *rnorm,
+                         const float *params, const void *packed, void *h,
+                         void *Q, float *a, int save, double *z, void *stream);
 int esr_gnn_backward(int L, int64_t n, int64_t nnz, const int32_t *rowptr,
                      const int32_t *col, const int32_t *rev,
                      const float *efeat, const float *vfeat, const double *r,
