@@ -202,6 +202,29 @@ __device__ __forceinline__ T sell_row(int64_t r, const int64_t *__restrict__ sof
   return y + src(r) * Av[r];
 }
 
+// A Linsys operator as a kernel argument: row(r, src) folds row r exactly as
+// linsys_row does. CsrRows reads the CSR; SellRows the SELL-64 form (rows
+// must be visited 64 at a time, wavefront-aligned).
+template <typename T> struct CsrRows {
+  const int32_t *rowptr;
+  const int32_t *col;
+  const T *Ae;
+  const T *Av;
+  template <typename Src> __device__ __forceinline__ T row(int64_t r, Src src) const {
+    return linsys_row<T>(r, rowptr, col, Ae, Av, src);
+  }
+};
+template <typename T, typename C> struct SellRows {
+  const int64_t *soff;
+  const uint8_t *cnt;
+  const C *colv;
+  const T *val;
+  const T *Av;
+  template <typename Src> __device__ __forceinline__ T row(int64_t r, Src src) const {
+    return sell_row<T>(r, soff, cnt, colv, val, Av, src);
+  }
+};
+
 template <typename T, typename Src>
 __device__ __forceinline__ T linsys_row_wave(int64_t r, const int32_t *__restrict__ rowptr,
                                              const int32_t *__restrict__ col,

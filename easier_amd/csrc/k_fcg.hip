@@ -34,17 +34,15 @@ struct FcgState {
 
 namespace {
 
-__global__ __launch_bounds__(kRB) void k_fcg_spmv(int64_t n, const int32_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ col,
-                                                 const double *__restrict__ Ae,
-                                                 const double *__restrict__ Av,
+template <typename Op>
+__global__ __launch_bounds__(kRB) void k_fcg_spmv(int64_t n, const Op A,
                                                  const double *__restrict__ p,
                                                  double *__restrict__ q, const FcgState *st,
                                                  double *part) {
   if (st->done) return;
   double acc[1] = {0.0};
   ESR_GRID_ROWS(i, n) {
-    const double y = linsys_row<double>(i, rowptr, col, Ae, Av, [&](int64_t c) { return p[c]; });
+    const double y = A.row(i, [&](int64_t c) { return p[c]; });
     q[i] = y;
     acc[0] += p[i] * y;
   }
@@ -107,14 +105,13 @@ __global__ __launch_bounds__(kRB) void k_fcg_dir(int64_t n, const double *__rest
 }
 
 // r = b - A x (float64); partials of ||r||^2 and ||b||^2
+template <typename Op>
 __global__ __launch_bounds__(kRB) void k_fcg_residual(
-    int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const double *__restrict__ Ae, const double *__restrict__ Av, const double *__restrict__ bv,
+    int64_t n, const Op A, const double *__restrict__ bv,
     const double *__restrict__ x, double *__restrict__ r, double *part) {
   double acc[2] = {0.0, 0.0};
   ESR_GRID_ROWS(i, n) {
-    const double ri = bv[i] - linsys_row<double>(i, rowptr, col, Ae, Av,
-                                                 [&](int64_t c) { return x[c]; });
+    const double ri = bv[i] - A.row(i, [&](int64_t c) { return x[c]; });
     r[i] = ri;
     acc[0] += ri * ri;
     acc[1] += bv[i] * bv[i];
@@ -197,23 +194,22 @@ ESR_API int esr_fcg_workspace(int64_t n, int64_t *state_bytes, int64_t *ws_bytes
   return ESR_OK;
 }
 
+namespace {
 // op codes: 0 spmv(p -> q), 1 update(x, r, ro, p, q), 2 dots(z, r, ro, d;
 // a0 = restart), 3 direction(z or r/d, p; a0 = restart), 4 residual(b, x -> r),
 // 5 jacobi(r -> z with d)
-ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
-                       const double *Ae, const double *Av, const double *d, const double *b,
-                       double *x, double *r, double *ro, double *p, double *q, double *z,
-                       int a0, void *state, void *ws, void *stream) {
-  ESR_GUARD_BEGIN
+template <typename Op>
+int fcg_op(int op, int64_t n, const Op &A, bool have_op, const double *d, const double *b,
+           double *x, double *r, double *ro, double *p, double *q, double *z, int a0,
+           void *state, void *ws, hipStream_t s) {
   ESR_REQUIRE(n >= 1 && state && ws, "esr_fcg_op: null argument");
-  hipStream_t s = as_stream(stream);
   FcgState *st = (FcgState *)state;
   double *part = (double *)ws;
   const unsigned g = red_grid(n);
   switch (op) {
   case 0:
-    ESR_REQUIRE(rowptr && col && Ae && Av && p && q, "esr_fcg_op spmv: null");
-    k_fcg_spmv<<<g, kRB, 0, s>>>(n, rowptr, col, Ae, Av, p, q, st, part);
+    ESR_REQUIRE(have_op && p && q, "esr_fcg_op spmv: null");
+    k_fcg_spmv<<<g, kRB, 0, s>>>(n, A, p, q, st, part);
     break;
   case 1:
     ESR_REQUIRE(x && r && ro && p && q, "esr_fcg_op update: null");
@@ -228,8 +224,8 @@ ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *
     k_fcg_dir<<<g, kRB, 0, s>>>(n, z, r, d, p, st, a0);
     break;
   case 4:
-    ESR_REQUIRE(rowptr && col && Ae && Av && b && x && r, "esr_fcg_op residual: null");
-    k_fcg_residual<<<g, kRB, 0, s>>>(n, rowptr, col, Ae, Av, b, x, r, part);
+    ESR_REQUIRE(have_op && b && x && r, "esr_fcg_op residual: null");
+    k_fcg_residual<<<g, kRB, 0, s>>>(n, A, b, x, r, part);
     break;
   case 5:
     ESR_REQUIRE(r && d && z, "esr_fcg_op jacobi: null");
@@ -243,6 +239,34 @@ ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *
     k_fcg_fin<<<1, kRB, 0, s>>>(op, g, part, st, a0);
   ESR_LAUNCHED();
   return ESR_OK;
+}
+} // namespace
+
+ESR_API int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
+                       const double *Ae, const double *Av, const double *d, const double *b,
+                       double *x, double *r, double *ro, double *p, double *q, double *z,
+                       int a0, void *state, void *ws, void *stream) {
+  ESR_GUARD_BEGIN
+  const CsrRows<double> A{rowptr, col, Ae, Av};
+  return fcg_op(op, n, A, rowptr && col && Ae && Av, d, b, x, r, ro, p, q, z, a0, state, ws,
+                as_stream(stream));
+  ESR_GUARD_END
+}
+
+// esr_fcg_op with the operator (ops 0 and 4) in its SELL-64 form.
+ESR_API int esr_fcg_op_sell(int op, int64_t n, const int64_t *soff, const uint8_t *cnt,
+                            const void *colv, int32_t idx16, const double *val,
+                            const double *Av, const double *d, const double *b, double *x,
+                            double *r, double *ro, double *p, double *q, double *z, int a0,
+                            void *state, void *ws, void *stream) {
+  ESR_GUARD_BEGIN
+  const bool have = soff && cnt && colv && val && Av;
+  if (idx16) {
+    const SellRows<double, int16_t> A{soff, cnt, (const int16_t *)colv, val, Av};
+    return fcg_op(op, n, A, have, d, b, x, r, ro, p, q, z, a0, state, ws, as_stream(stream));
+  }
+  const SellRows<double, int32_t> A{soff, cnt, (const int32_t *)colv, val, Av};
+  return fcg_op(op, n, A, have, d, b, x, r, ro, p, q, z, a0, state, ws, as_stream(stream));
   ESR_GUARD_END
 }
 

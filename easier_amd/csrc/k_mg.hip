@@ -224,17 +224,14 @@ __global__ __launch_bounds__(kMB) void k_mg_prolong(int64_t nf, const int32_t *_
 // Damped-Jacobi pre-smoothing from z = 0 and the residual it leaves:
 // z = wd r; t = r - A z, with wd = w / d per row (precomputed, so the sweep
 // multiplies instead of dividing at every neighbour).
-template <typename T>
-__global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ col,
-                                                      const T *__restrict__ Ae,
-                                                      const T *__restrict__ Av,
+template <typename T, typename Op>
+__global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const Op A,
                                                       const T *__restrict__ wd,
                                                       const T *__restrict__ r, T *__restrict__ z,
                                                       T *__restrict__ t) {
   const int64_t i = mg_tid();
   if (i >= n) return;
-  const T az = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return wd[j] * r[j]; });
+  const T az = A.row(i, [&](int64_t j) { return wd[j] * r[j]; });
   z[i] = wd[i] * r[i];
   t[i] = r[i] - az;
 }
@@ -244,13 +241,9 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const int32_t 
 // correction already added by k_mg_prolong): the fused form evaluates P at
 // every neighbour (a rowptr -> col -> parent -> zc load chain), which runs
 // about 3x slower on the finest level than the two-pass form.
-template <typename T>
+template <typename T, typename Op>
 __global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_t *__restrict__ par,
-                                                       const int32_t *__restrict__ rowptr,
-                                                       const int32_t *__restrict__ col,
-                                                       const T *__restrict__ Ae,
-                                                       const T *__restrict__ Av,
-                                                       const T *__restrict__ wd,
+                                                       const Op A, const T *__restrict__ wd,
                                                        const T *__restrict__ r,
                                                        const T *__restrict__ z,
                                                        const T *__restrict__ zc,
@@ -259,23 +252,19 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_
   if (i >= nv) return;
   if (zc) {
     auto y = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
-    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, y);
+    const T ay = A.row(i, y);
     const T yi = y(i);
     out[i] = yi + wd[i] * (r[i] - ay);
   } else {
-    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return z[j]; });
+    const T ay = A.row(i, [&](int64_t j) { return z[j]; });
     out[i] = z[i] + wd[i] * (r[i] - ay);
   }
 }
 
 // t = r - A y with y = z (zc null) or y = z + P zc (then also stored).
-template <typename T>
+template <typename T, typename Op>
 __global__ __launch_bounds__(kMB) void k_mg_residual(int64_t n, int64_t nf,
-                                                    const int32_t *__restrict__ par,
-                                                    const int32_t *__restrict__ rowptr,
-                                                    const int32_t *__restrict__ col,
-                                                    const T *__restrict__ Ae,
-                                                    const T *__restrict__ Av,
+                                                    const int32_t *__restrict__ par, const Op A,
                                                     const T *__restrict__ r,
                                                     const T *__restrict__ z,
                                                     const T *__restrict__ zc, T *__restrict__ y,
@@ -284,11 +273,11 @@ __global__ __launch_bounds__(kMB) void k_mg_residual(int64_t n, int64_t nf,
   if (i >= n) return;
   if (zc) {
     auto yv = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
-    const T ay = linsys_row<T>(i, rowptr, col, Ae, Av, yv);
+    const T ay = A.row(i, yv);
     y[i] = yv(i);
     t[i] = r[i] - ay;
   } else {
-    const T az = linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t j) { return z[j]; });
+    const T az = A.row(i, [&](int64_t j) { return z[j]; });
     t[i] = r[i] - az;
   }
 }
@@ -400,17 +389,61 @@ ESR_API int esr_mg_prolong(int dtype, int64_t nf, const int32_t *par, const void
   ESR_GUARD_END
 }
 
+namespace {
+template <typename T, typename Op>
+int mg_pre(int64_t n, const Op &A, const void *wd, const void *r, void *z, void *t,
+           hipStream_t s) {
+  k_mg_jacobi_pre<T><<<mg_grid(n), kMB, 0, s>>>(n, A, (const T *)wd, (const T *)r, (T *)z,
+                                                (T *)t);
+  ESR_LAUNCHED();
+  return ESR_OK;
+}
+template <typename T, typename Op>
+int mg_post(int64_t nf, const int32_t *par, const Op &A, const void *wd, const void *r,
+            const void *z, const void *zc, void *out, hipStream_t s) {
+  const int64_t nv = (nf + 1) * (nf + 1);
+  k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, s>>>(nf, par, A, (const T *)wd, (const T *)r,
+                                                  (const T *)z, (const T *)zc, (T *)out);
+  ESR_LAUNCHED();
+  return ESR_OK;
+}
+template <typename T, typename Op>
+int mg_res(int64_t n, int64_t nf, const int32_t *par, const Op &A, const void *r, const void *z,
+           const void *zc, void *y, void *t, hipStream_t s) {
+  k_mg_residual<T><<<mg_grid(n), kMB, 0, s>>>(n, nf, par, A, (const T *)r, (const T *)z,
+                                              (const T *)zc, (T *)y, (T *)t);
+  ESR_LAUNCHED();
+  return ESR_OK;
+}
+// the operator of a sweep: CSR, or SELL-64 with int16 / int32 columns
+#define MG_OP_DISPATCH(T, CALL_WITH_A)                                                     \
+  do {                                                                                     \
+    if (soff == nullptr) {                                                                 \
+      const CsrRows<T> A{rowptr, col, (const T *)Ae, (const T *)Av};                       \
+      return CALL_WITH_A;                                                                  \
+    } else if (idx16) {                                                                    \
+      const SellRows<T, int16_t> A{soff, cnt, (const int16_t *)colv, (const T *)val,       \
+                                   (const T *)Av};                                         \
+      return CALL_WITH_A;                                                                  \
+    } else {                                                                               \
+      const SellRows<T, int32_t> A{soff, cnt, (const int32_t *)colv, (const T *)val,       \
+                                   (const T *)Av};                                         \
+      return CALL_WITH_A;                                                                  \
+    }                                                                                      \
+  } while (0)
+} // namespace
+
 ESR_API int esr_mg_jacobi_pre(int dtype, int64_t n, const int32_t *rowptr, const int32_t *col,
                               const void *Ae, const void *Av, const void *wd, const void *r,
                               void *z, void *t, void *stream) {
   ESR_GUARD_BEGIN
   ESR_REQUIRE(n >= 1, "esr_mg_jacobi_pre: n must be >= 1");
   ESR_REQUIRE(rowptr && col && Ae && Av && wd && r && z && t, "esr_mg_jacobi_pre: null pointer");
-  MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_pre<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
-                               n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)wd,
-                               (const T *)r, (T *)z, (T *)t)));
-  ESR_LAUNCHED();
-  return ESR_OK;
+  const int64_t *soff = nullptr;
+  const uint8_t *cnt = nullptr;
+  const void *colv = nullptr, *val = nullptr;
+  const int idx16 = 0;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_pre<T>(n, A, wd, r, z, t, as_stream(stream)))));
   ESR_GUARD_END
 }
 
@@ -423,12 +456,12 @@ ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *par, const 
   ESR_REQUIRE(rowptr && col && Ae && Av && wd && r && z && out && (par || !zc),
               "esr_mg_jacobi_post: null pointer");
   ESR_REQUIRE(out != z, "esr_mg_jacobi_post: out must not alias z");
-  const int64_t nv = (nf + 1) * (nf + 1);
-  MG_DTYPE_DISPATCH(dtype, (k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, as_stream(stream)>>>(
-                               nf, par, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)wd,
-                               (const T *)r, (const T *)z, (const T *)zc, (T *)out)));
-  ESR_LAUNCHED();
-  return ESR_OK;
+  const int64_t *soff = nullptr;
+  const uint8_t *cnt = nullptr;
+  const void *colv = nullptr, *val = nullptr;
+  const int idx16 = 0;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nf, par, A, wd, r, z, zc, out,
+                                                         as_stream(stream)))));
   ESR_GUARD_END
 }
 
@@ -446,11 +479,68 @@ ESR_API int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *par
                 "esr_mg_residual: n=%lld is not the vertex count of an even grid nf=%lld",
                 (long long)n, (long long)nf);
   }
-  MG_DTYPE_DISPATCH(dtype, (k_mg_residual<T><<<mg_grid(n), kMB, 0, as_stream(stream)>>>(
-                               n, nf, par, rowptr, col, (const T *)Ae, (const T *)Av,
-                               (const T *)r, (const T *)z, (const T *)zc, (T *)y, (T *)t)));
-  ESR_LAUNCHED();
-  return ESR_OK;
+  const int64_t *soff = nullptr;
+  const uint8_t *cnt = nullptr;
+  const void *colv = nullptr, *val = nullptr;
+  const int idx16 = 0;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_res<T>(n, nf, par, A, r, z, zc, y, t,
+                                                        as_stream(stream)))));
+  ESR_GUARD_END
+}
+
+// The three sweeps on the level operator's SELL-64 form (esr_sell_plan /
+// esr_sell_fill, values in the sweep's dtype): the same results.
+ESR_API int esr_mg_jacobi_pre_sell(int dtype, int64_t n, const int64_t *soff, const uint8_t *cnt,
+                                   const void *colv, int32_t idx16, const void *val,
+                                   const void *Av, const void *wd, const void *r, void *z,
+                                   void *t, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_jacobi_pre_sell: n must be >= 1");
+  ESR_REQUIRE(soff && cnt && colv && val && Av && wd && r && z && t,
+              "esr_mg_jacobi_pre_sell: null pointer");
+  const int32_t *rowptr = nullptr, *col = nullptr;
+  const void *Ae = nullptr;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_pre<T>(n, A, wd, r, z, t, as_stream(stream)))));
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_jacobi_post_sell(int dtype, int64_t nf, const int32_t *par,
+                                    const int64_t *soff, const uint8_t *cnt, const void *colv,
+                                    int32_t idx16, const void *val, const void *Av,
+                                    const void *wd, const void *r, const void *z,
+                                    const void *zc, void *out, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 2 && nf % 2 == 0, "esr_mg_jacobi_post_sell: fine grid n=%lld is not even",
+              (long long)nf);
+  ESR_REQUIRE(soff && cnt && colv && val && Av && wd && r && z && out && (par || !zc),
+              "esr_mg_jacobi_post_sell: null pointer");
+  ESR_REQUIRE(out != z, "esr_mg_jacobi_post_sell: out must not alias z");
+  const int32_t *rowptr = nullptr, *col = nullptr;
+  const void *Ae = nullptr;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nf, par, A, wd, r, z, zc, out,
+                                                         as_stream(stream)))));
+  ESR_GUARD_END
+}
+
+ESR_API int esr_mg_residual_sell(int dtype, int64_t n, int64_t nf, const int32_t *par,
+                                 const int64_t *soff, const uint8_t *cnt, const void *colv,
+                                 int32_t idx16, const void *val, const void *Av, const void *r,
+                                 const void *z, const void *zc, void *y, void *t, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_mg_residual_sell: n must be >= 1");
+  ESR_REQUIRE(soff && cnt && colv && val && Av && r && z && t,
+              "esr_mg_residual_sell: null pointer");
+  if (zc) {
+    ESR_REQUIRE(par && y && y != z,
+                "esr_mg_residual_sell: with zc, par and y must be given and y must not alias z");
+    ESR_REQUIRE(nf >= 2 && nf % 2 == 0 && (nf + 1) * (nf + 1) == n,
+                "esr_mg_residual_sell: n=%lld is not the vertex count of an even grid nf=%lld",
+                (long long)n, (long long)nf);
+  }
+  const int32_t *rowptr = nullptr, *col = nullptr;
+  const void *Ae = nullptr;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_res<T>(n, nf, par, A, r, z, zc, y, t,
+                                                        as_stream(stream)))));
   ESR_GUARD_END
 }
 

@@ -30,6 +30,7 @@ damped Jacobi in the other: on the 2048 x 2048 variable-coefficient problem
 that took 10 flexible-CG steps, the network in both positions 15.
 """
 import ctypes
+import types
 from typing import List, Optional
 
 import numpy as np
@@ -50,6 +51,11 @@ nat.register_signatures({
     'esr_mg_jacobi_post': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 11),
     'esr_mg_residual': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 11),
     'esr_mg_dense': (_c_i32, [_c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp]),
+    'esr_mg_jacobi_pre_sell': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 3 + [_c_i32] + [_c_vp] * 7),
+    'esr_mg_jacobi_post_sell': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 4 + [_c_i32]
+                                + [_c_vp] * 8),
+    'esr_mg_residual_sell': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 4 + [_c_i32]
+                             + [_c_vp] * 8),
 })
 
 
@@ -192,6 +198,11 @@ class MultigridPreconditioner:
             # the operator the cycle reads (a float32 copy in a float32 cycle)
             L.cAe = L.Ae if cd == torch.float64 else L.Ae.to(cd)
             L.cAv = L.Av if cd == torch.float64 else L.Av.to(cd)
+            L.sell = None
+            if L.Ainv is None:
+                from ..solver.fused import make_sell
+                L.sell = make_sell(types.SimpleNamespace(rowptr=L.rowptr, col=L.col, Ae=L.cAe,
+                                                         Av=L.cAv, n=L.nv))
             L.wd = ops.ewise('div', self.omega, L.Av)         # w / d, float64
             if cd != torch.float64:
                 L.wd = L.wd.to(cd)
@@ -256,6 +267,27 @@ class MultigridPreconditioner:
     def __call__(self, r):
         return self.apply(r, torch.empty_like(r))
 
+    # sweeps on a level's operator: its SELL form when it has one, else CSR
+    def _pre(self, L, r, z, t):
+        sp, DT = nat.stream_ptr(), self.dt
+        if L.sell is not None:
+            nat.call('esr_mg_jacobi_pre_sell', DT, L.nv, *L.sell.arrays(), L.cAv.data_ptr(),
+                     L.wd.data_ptr(), r.data_ptr(), z.data_ptr(), t.data_ptr(), sp)
+        else:
+            nat.call('esr_mg_jacobi_pre', DT, L.nv, L.rowptr.data_ptr(), L.col.data_ptr(),
+                     L.cAe.data_ptr(), L.cAv.data_ptr(), L.wd.data_ptr(), r.data_ptr(),
+                     z.data_ptr(), t.data_ptr(), sp)
+
+    def _residual(self, L, r, z, t):
+        sp, DT = nat.stream_ptr(), self.dt
+        if L.sell is not None:
+            nat.call('esr_mg_residual_sell', DT, L.nv, L.n_grid, None, *L.sell.arrays(),
+                     L.cAv.data_ptr(), r.data_ptr(), z.data_ptr(), None, None, t.data_ptr(), sp)
+        else:
+            nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, L.rowptr.data_ptr(),
+                     L.col.data_ptr(), L.cAe.data_ptr(), L.cAv.data_ptr(), r.data_ptr(),
+                     z.data_ptr(), None, None, t.data_ptr(), sp)
+
     def _cycle(self, k, r, out):
         L = self.levels[k]
         sp = nat.stream_ptr()
@@ -266,40 +298,41 @@ class MultigridPreconditioner:
             return
         C = self.levels[k + 1]
         P = lambda t: None if t is None else t.data_ptr()
-        csr = (L.rowptr.data_ptr(), L.col.data_ptr(), L.cAe.data_ptr(), L.cAv.data_ptr())
         if L.graph is None:
-            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, L.wd.data_ptr(), r.data_ptr(),
-                     L.z.data_ptr(), L.t.data_ptr(), sp)
+            self._pre(L, r, L.z, L.t)
             nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(),
                      C.r.data_ptr(), sp)
             self._cycle(k + 1, C.r, C.zo)
-            self._post(L, C, csr, r, out)
+            self._post(L, C, r, out)
             return
         S = self.smoother
         if self.learned_pre:
             S.forward_into(L.graph, L.bufs, r, L.z)
-            nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, *csr, r.data_ptr(),
-                     L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
+            self._residual(L, r, L.z, L.t)
         else:
-            nat.call('esr_mg_jacobi_pre', DT, L.nv, *csr, L.wd.data_ptr(), r.data_ptr(),
-                     L.z.data_ptr(), L.t.data_ptr(), sp)
+            self._pre(L, r, L.z, L.t)
         nat.call('esr_mg_restrict', DT, L.n_grid, P(L.fidx), L.t.data_ptr(), C.r.data_ptr(),
                  sp)
         self._cycle(k + 1, C.r, C.zo)
         if self.learned_post:
             nat.call('esr_mg_prolong', DT, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
                      L.z.data_ptr(), sp)
-            nat.call('esr_mg_residual', DT, L.nv, L.n_grid, None, *csr, r.data_ptr(),
-                     L.z.data_ptr(), None, None, L.t.data_ptr(), sp)
+            self._residual(L, r, L.z, L.t)
             S.forward_into(L.graph, L.bufs, L.t, L.s)
             ops.ewise('add', L.z, L.s, out=out)
         else:
-            self._post(L, C, csr, r, out)
+            self._post(L, C, r, out)
 
-    def _post(self, L, C, csr, r, out):
+    def _post(self, L, C, r, out):
         """z += P zc (in place), then out = z + w (r - A z) / d."""
         sp = nat.stream_ptr()
         nat.call('esr_mg_prolong', self.dt, L.n_grid, L.par.data_ptr(), C.zo.data_ptr(),
                  L.z.data_ptr(), sp)
-        nat.call('esr_mg_jacobi_post', self.dt, L.n_grid, None, *csr, L.wd.data_ptr(),
-                 r.data_ptr(), L.z.data_ptr(), None, out.data_ptr(), sp)
+        if L.sell is not None:
+            nat.call('esr_mg_jacobi_post_sell', self.dt, L.n_grid, None, *L.sell.arrays(),
+                     L.cAv.data_ptr(), L.wd.data_ptr(), r.data_ptr(), L.z.data_ptr(), None,
+                     out.data_ptr(), sp)
+        else:
+            nat.call('esr_mg_jacobi_post', self.dt, L.n_grid, None, L.rowptr.data_ptr(),
+                     L.col.data_ptr(), L.cAe.data_ptr(), L.cAv.data_ptr(), L.wd.data_ptr(),
+                     r.data_ptr(), L.z.data_ptr(), None, out.data_ptr(), sp)

@@ -28,13 +28,15 @@ from ...core import module as esr
 from ...core import ops
 from ...core.utils import logger
 from ..preconditioner.learned import LearnedPreconditioner, MeshGraph
-from .fused import LinsysCSR
+from .fused import FusedCG, LinsysCSR, make_sell
 from .mixed import TrueResidual
 
 _c_i32, _c_i64, _c_vp, _c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
 nat.register_signatures({
     'esr_fcg_workspace': (_c_i32, [_c_i64, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
     'esr_fcg_op': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 12 + [_c_i32, _c_vp, _c_vp, _c_vp]),
+    'esr_fcg_op_sell': (_c_i32, [_c_i32, _c_i64] + [_c_vp] * 3 + [_c_i32] + [_c_vp] * 10
+                        + [_c_i32, _c_vp, _c_vp, _c_vp]),
     'esr_fcg_set': (_c_i32, [_c_vp, _c_dbl, _c_i64, _c_vp]),
     'esr_fcg_read': (_c_i32, [_c_vp, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(_c_i64), _c_vp]),
@@ -66,12 +68,27 @@ class FcgKernels:
         self.pcg_ws = torch.empty(wb.value, dtype=torch.uint8, device=device)
         self.chunk = chunk
         self.graphs = {}
+        # the operator's SELL-64 form (csrc/esr_krylov.h) for the products,
+        # built on the first solve and refreshed from Ae on every solve
+        self.sell, self.want_sell = None, FusedCG.use_sell
+
+    def _refresh_operator(self):
+        if self.want_sell:
+            self.want_sell = False
+            self.sell = make_sell(self)
+            self.graphs.clear()
+        elif self.sell is not None:
+            self.sell.refresh(self.Ae)
 
     def op(self, code, b=None, x=None, a0=0):
         P = lambda t: None if t is None else t.data_ptr()
-        nat.call('esr_fcg_op', code, self.n, P(self.rowptr), P(self.col), P(self.Ae),
-                 P(self.Av), P(self.Av), P(b), P(x), P(self.r), P(self.ro), P(self.p),
-                 P(self.q), P(self.z), a0, P(self.state), P(self.ws), nat.stream_ptr())
+        vecs = [P(self.Av), P(b), P(x), P(self.r), P(self.ro), P(self.p), P(self.q),
+                P(self.z), a0, P(self.state), P(self.ws), nat.stream_ptr()]
+        if self.sell is not None:
+            nat.call('esr_fcg_op_sell', code, self.n, *self.sell.arrays(), P(self.Av), *vecs)
+        else:
+            nat.call('esr_fcg_op', code, self.n, P(self.rowptr), P(self.col), P(self.Ae),
+                     P(self.Av), *vecs)
 
     def read(self):
         sc = (ctypes.c_double * 9)()
@@ -115,8 +132,13 @@ class FcgKernels:
                  -1 if maxiter is None else int(maxiter), nat.stream_ptr())
         if 'pcg' not in self.graphs:
             def launch(sp):
-                nat.call('esr_pcg_steps', nat.F64, self.n, *csr, P(x), P(self.r), P(self.z),
-                         P(self.p), P(self.q), P(self.pcg_state), P(self.pcg_ws), 32, sp)
+                vecs = [P(x), P(self.r), P(self.z), P(self.p), P(self.q), P(self.pcg_state),
+                        P(self.pcg_ws), 32, sp]
+                if self.sell is not None:
+                    nat.call('esr_pcg_steps_sell', nat.F64, self.n, *self.sell.arrays(),
+                             P(self.Av), P(self.Av), *vecs)
+                else:
+                    nat.call('esr_pcg_steps', nat.F64, self.n, *csr, *vecs)
             self.graphs['pcg'] = (_Graph(launch), x.data_ptr())
         g, xp = self.graphs['pcg']
         assert xp == x.data_ptr()
@@ -130,6 +152,7 @@ class FcgKernels:
 
     def solve(self, b, x, M, rtol=1e-8, atol=None, maxiter=None, window=200,
               debug_iter=None):
+        self._refresh_operator()
         self.op(RESIDUAL, b=b, x=x)
         s = self.read()
         bnorm = float(np.sqrt(s['bb']))

@@ -465,6 +465,13 @@ int esr_fcg_op(int op, int64_t n, const int32_t *rowptr, const int32_t *col,
                const double *b, double *x, double *r, double *ro, double *p,
                double *q, double *z, int a0, void *state, void *ws,
                void *stream);
+/* esr_fcg_op with the operator of ops 0 (product) and 4 (residual) in its
+ * SELL-64 form; the same results.                                           */
+int esr_fcg_op_sell(int op, int64_t n, const int64_t *soff, const uint8_t *cnt,
+                    const void *colv, int32_t idx16, const double *val,
+                    const double *Av, const double *d, const double *b,
+                    double *x, double *r, double *ro, double *p, double *q,
+                    double *z, int a0, void *state, void *ws, void *stream);
 int esr_fcg_set(void *state, double tol2, int64_t maxit, void *stream);
 int esr_fcg_read(const void *state, double *scalars, int64_t *ints, void *stream);
 
@@ -517,6 +524,24 @@ int esr_mg_residual(int dtype, int64_t n, int64_t nf, const int32_t *par,
                     const int32_t *rowptr, const int32_t *col, const void *Ae,
                     const void *Av, const void *r, const void *z, const void *zc,
                     void *y, void *t, void *stream);
+/* The three sweeps above with the level operator in its SELL-64 form
+ * (esr_sell_plan / esr_sell_fill; values in the sweep's dtype): the same
+ * results. No reference counterpart (multigrid is this package's own). */
+int esr_mg_jacobi_pre_sell(int dtype, int64_t n, const int64_t *soff,
+                           const uint8_t *cnt, const void *colv, int32_t idx16,
+                           const void *val, const void *Av, const void *wd,
+                           const void *r, void *z, void *t, void *stream);
+int esr_mg_jacobi_post_sell(int dtype, int64_t nf, const int32_t *par,
+                            const int64_t *soff, const uint8_t *cnt,
+                            const void *colv, int32_t idx16, const void *val,
+                            const void *Av, const void *wd, const void *r,
+                            const void *z, const void *zc, void *out,
+                            void *stream);
+int esr_mg_residual_sell(int dtype, int64_t n, int64_t nf, const int32_t *par,
+                         const int64_t *soff, const uint8_t *cnt,
+                         const void *colv, int32_t idx16, const void *val,
+                         const void *Av, const void *r, const void *z,
+                         const void *zc, void *y, void *t, void *stream);
 int esr_mg_dense(int dtype, int64_t n, const double *Ainv, const void *r, void *z,
                  void *stream);
 
