@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kMaxK = 64;
 constexpr int kMaxGridM = 512;
+constexpr int kMaxGridSpmvM = 8192;
 
 struct Lanes {
   int L;        // lanes per row (power of two >= K)
@@ -119,12 +120,13 @@ __global__ __launch_bounds__(kRB) void k_m_init_fin(int K, int64_t g, const doub
   }
 }
 
-// Step part 1 (cg.py:44-47): AP = A P; partials of p.Ap. (On 64 right-hand
-// sides of the 1024 x 1024 mesh this kernel takes ~0.95 ms of a 2.1 ms
-// step: every row gathers 7 neighbour rows of 512 bytes, and the rows
-// +-1025 away are dealt to other XCDs. One contiguous range of rows per XCD
-// measured slower, 6.61 vs 6.48 s per solve, as did grid caps of 2,048 and
-// 8,192 workgroups.)
+// Step part 1 (cg.py:44-47): AP = A P; partials of p.Ap. On 64 right-hand
+// sides of the 1024 x 1024 mesh this kernel took 0.91 ms of a 1.9 ms step
+// on the 512-workgroup grid of the other kernels (1.3 TB/s on its 1.16 GB:
+// a wavefront serves one row, 513 rows in turn); on its own grid of up to
+// 8,192 workgroups the solve takes 6.03 s instead of 6.48 s (2,048: 6.21 s,
+// 32,768: 6.12 s). One contiguous range of rows per XCD measured slower
+// (6.61 vs 6.48 s), as did two rows per wavefront iteration (6.43 s).)
 __global__ __launch_bounds__(kRB) void k_m_spmv(int64_t n, int K, const int32_t *__restrict__ rowptr,
                                                const int32_t *__restrict__ col,
                                                const double *__restrict__ Ae,
@@ -230,12 +232,18 @@ __global__ __launch_bounds__(kRB) void k_m_p(int64_t n, int K, const double *__r
   }
 }
 
-inline unsigned mrhs_grid(int64_t n, int K) {
+inline unsigned mrhs_grid_cap(int64_t n, int K, int64_t cap) {
   int L = 1;
   while (L < K) L <<= 1;
   const int64_t g = ceil_div(n, kRB / L);
-  return (unsigned)(g < kMaxGridM ? (g > 0 ? g : 1) : kMaxGridM);
+  return (unsigned)(g < cap ? (g > 0 ? g : 1) : cap);
 }
+inline unsigned mrhs_grid(int64_t n, int K) { return mrhs_grid_cap(n, K, kMaxGridM); }
+// The product's grid: a wavefront serves one row at a time through dependent
+// row-pointer -> column -> neighbour-row loads, so it needs many wavefronts in
+// flight; its partials (one per workgroup and right-hand side) are summed by
+// k_m_alpha in a fixed order whatever their number.
+inline unsigned mrhs_spmv_grid(int64_t n, int K) { return mrhs_grid_cap(n, K, kMaxGridSpmvM); }
 
 } // namespace
 } // namespace esr
@@ -246,7 +254,8 @@ ESR_API int esr_mrhs_sizes(int64_t n, int64_t K, int64_t *state_bytes, int64_t *
   ESR_REQUIRE(n >= 1 && K >= 1 && K <= kMaxK && state_bytes && ws_bytes,
               "esr_mrhs_sizes: need n >= 1 and 1 <= K <= %d", kMaxK);
   *state_bytes = K * (int64_t)sizeof(MrhsState);
-  *ws_bytes = 2 * K * (int64_t)mrhs_grid(n, (int)K) * (int64_t)sizeof(double);
+  const int64_t g = mrhs_grid(n, (int)K), g1 = mrhs_spmv_grid(n, (int)K);
+  *ws_bytes = K * (2 * g > g1 ? 2 * g : g1) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -279,11 +288,11 @@ ESR_API int esr_mrhs_steps(int64_t n, int64_t K, const int32_t *rowptr, const in
   ESR_REQUIRE(rowptr && col && Ae && Av && X && R && Z && P && AP && state && ws && active,
               "esr_mrhs_steps: null argument");
   hipStream_t s = as_stream(stream);
-  const unsigned g = mrhs_grid(n, (int)K);
+  const unsigned g = mrhs_grid(n, (int)K), g1 = mrhs_spmv_grid(n, (int)K);
   MrhsState *st = (MrhsState *)state;
   for (int64_t it = 0; it < count; ++it) {
-    k_m_spmv<<<g, kRB, 0, s>>>(n, (int)K, rowptr, col, Ae, Av, P, AP, st, ws);
-    k_m_alpha<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, st);
+    k_m_spmv<<<g1, kRB, 0, s>>>(n, (int)K, rowptr, col, Ae, Av, P, AP, st, ws);
+    k_m_alpha<<<(unsigned)K, kRB, 0, s>>>((int)K, g1, ws, st);
     k_m_xr<<<g, kRB, 0, s>>>(n, (int)K, diag, X, R, Z, P, AP, st, ws);
     k_m_test<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, st, active);
     k_m_p<<<g, kRB, 0, s>>>(n, (int)K, Z, P, st);
