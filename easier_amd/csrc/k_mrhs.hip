@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kRB) void k_m_init_fin(int K, int64_t g, const doub
 // a wavefront serves one row, 513 rows in turn); on its own grid of up to
 // 8,192 workgroups the solve takes 6.03 s instead of 6.48 s (2,048: 6.21 s,
 // 32,768: 6.12 s). One contiguous range of rows per XCD measured slower
-// (6.61 vs 6.48 s), as did two rows per wavefront iteration (6.43 s).)
+// (6.61 vs 6.48 s), as did two rows per wavefront iteration (6.43 s).
 __global__ __launch_bounds__(kRB) void k_m_spmv(int64_t n, int K, const int32_t *__restrict__ rowptr,
                                                const int32_t *__restrict__ col,
                                                const double *__restrict__ Ae,
