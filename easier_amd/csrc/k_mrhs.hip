@@ -125,8 +125,11 @@ __global__ __launch_bounds__(kRB) void k_m_init_fin(int K, int64_t g, const doub
 // on the 512-workgroup grid of the other kernels (1.3 TB/s on its 1.16 GB:
 // a wavefront serves one row, 513 rows in turn); on its own grid of up to
 // 8,192 workgroups the solve takes 6.03 s instead of 6.48 s (2,048: 6.21 s,
-// 32,768: 6.12 s). One contiguous range of rows per XCD measured slower
-// (6.61 vs 6.48 s), as did two rows per wavefront iteration (6.43 s).
+// 32,768: 6.12 s). It fetches 2.2 GB through the L2s for 0.63 GB of
+// compulsory reads (neighbour rows fetched by several XCDs), but a
+// contiguous chunk of rows per workgroup with the chunks of one XCD
+// consecutive did not shorten the solve (6.08 s), nor did two rows per
+// wavefront iteration (6.43 s on the 512-workgroup grid).
 __global__ __launch_bounds__(kRB) void k_m_spmv(int64_t n, int K, const int32_t *__restrict__ rowptr,
                                                const int32_t *__restrict__ col,
                                                const double *__restrict__ Ae,
