@@ -41,6 +41,13 @@ struct GmresState {
   double H[33 * 32];  // (m+1) x m, row-major with row stride 32 (m <= 32)
   int64_t broken;     // Arnoldi breakdown (h < 1e-15) seen at step jend
   int64_t jend;       // last Arnoldi step of the cycle
+  // restart loop on the device (esr_gmres_loop_*): exit tolerance and cap,
+  // cycles run, exit flag, and the cycle's least-squares solution
+  double tol;
+  int64_t maxiter;    // < 0: none
+  int64_t iters;
+  int64_t done;
+  double y[32];
 };
 
 constexpr int kGmresMaxRestart = 32;

@@ -376,7 +376,9 @@ template <typename T>
 __global__ __launch_bounds__(kRB) void k_gm_residual(
     int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const T *__restrict__ Ae, const T *__restrict__ Av, const T *__restrict__ bvec,
-    const T *__restrict__ x, T *__restrict__ r, double *__restrict__ part) {
+    const T *__restrict__ x, T *__restrict__ r, double *__restrict__ part,
+    const GmresState *loop) {
+  if (loop && loop->done) return;
   double acc[1] = {0.0};
   ESR_GRID_ROWS(i, n) {
     const T ri = bvec[i] - linsys_row<T>(i, rowptr, col, Ae, Av, [&](int64_t c) { return x[c]; });
@@ -664,6 +666,121 @@ __global__ __launch_bounds__(kRB) void k_gm_update_x(int64_t n, int jc, const T 
   ESR_GRID_ROWS(r, n) {
     T dx = T(0);
     for (int i = 0; i < jc; ++i) dx = dx + V[(int64_t)i * n + r] * y[i];
+    x[r] = x[r] + (diag ? dx / diag[r] : dx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The restart loop of gmres.py:214-266 on the device (esr_gmres_loop_*): the
+// loop test after each true residual, the cycle's least-squares problem and
+// the update, so a captured graph can run several restart cycles with no
+// host round trip. Every kernel is a no-op once `done` is set.
+// ---------------------------------------------------------------------------
+// tol = max(rtol ||b||, atol) if atol > 0 else rtol ||b|| (gmres.py's solve)
+__global__ void k_gm_loop_arm(GmresState *st, double rtol, double atol, int64_t maxiter) {
+  double tol = rtol * st->bnorm;
+  if (atol > 0.0 && atol > tol) tol = atol;
+  st->tol = tol;
+  st->maxiter = maxiter;
+  st->iters = 0;
+  st->done = 0;
+}
+
+// ||r|| of the residual kernel's partials, then the loop test: exit when
+// ||r|| <= tol, at the cycle cap, or on a NaN residual without a cap; else
+// count the cycle about to run and clear the breakdown flag. On exit the
+// breakdown flag is set as well, which makes the Arnoldi kernels no-ops.
+// (Every thread reads `done` before the first barrier; thread 0 writes only
+// after the barriers of sum_partials.)
+__global__ __launch_bounds__(kRB) void k_gm_residual_fin_loop(int64_t g,
+                                                             const double *__restrict__ part,
+                                                             GmresState *st) {
+  if (st->done) return;
+  double tot[1];
+  sum_partials<1>(part, g, 0, g, tot);
+  if (threadIdx.x == 0) {
+    const double rn = cr_sqrt(tot[0]);
+    st->rnorm = rn;
+    st->jend = -1;
+    const bool conv = rn == rn && rn <= st->tol;
+    const bool capped = st->maxiter >= 0 && st->iters >= st->maxiter;
+    const bool nan_stop = rn != rn && st->maxiter < 0;
+    if (conv || capped || nan_stop) {
+      st->done = 1;
+      st->broken = 1;
+    } else {
+      st->iters += 1;
+      st->broken = 0;
+    }
+  }
+}
+
+// min || rnorm e_0 - H y || over the cycle's jend + 1 columns (rows 0 ..
+// jend + 1 of the upper Hessenberg H): Givens rotations zero the
+// subdiagonal one column at a time, then back substitution. One wavefront;
+// lane k holds column k. (The reference solves the same least-squares
+// problem by a thin SVD, gmres.py:255-258; both give its solution up to
+// rounding when H has full column rank. A zero pivot gives y_i = 0.)
+__global__ __launch_bounds__(64) void k_gm_lstsq(GmresState *st) {
+  if (st->done) return;
+  __shared__ double R[kGmresMaxRestart + 1][kGmresMaxRestart + 1];
+  __shared__ double gv[kGmresMaxRestart + 1];
+  __shared__ double cs[2];
+  const int k = threadIdx.x;
+  const int nc = (int)st->jend + 1;
+  if (nc <= 0) return;
+  for (int i = 0; i <= nc; ++i)
+    if (k < nc) R[i][k] = st->H[i * kGmresMaxRestart + k];
+  if (k <= nc) gv[k] = k == 0 ? st->rnorm : 0.0;
+  __syncthreads();
+  for (int i = 0; i < nc; ++i) {
+    if (k == 0) {
+      const double a = R[i][i], b = R[i + 1][i];
+      const double h = sqrt(a * a + b * b);
+      cs[0] = h > 0.0 ? a / h : 1.0;
+      cs[1] = h > 0.0 ? b / h : 0.0;
+    }
+    __syncthreads();
+    const double c = cs[0], sn = cs[1];
+    if (k >= i && k < nc) {
+      const double top = R[i][k], bot = R[i + 1][k];
+      R[i][k] = c * top + sn * bot;
+      R[i + 1][k] = -sn * top + c * bot;
+    }
+    if (k == 0) {
+      const double gi = gv[i];
+      gv[i] = c * gi;
+      gv[i + 1] = -sn * gi;
+    }
+    __syncthreads();
+  }
+  __shared__ double y[kGmresMaxRestart];
+  for (int i = nc - 1; i >= 0; --i) {
+    double t = (k > i && k < nc) ? R[i][k] * y[k] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (k == 0) {
+      const double d = R[i][i];
+      y[i] = d != 0.0 ? (gv[i] - t) / d : 0.0;
+    }
+    __syncthreads();
+  }
+  if (k < nc) st->y[k] = y[k];
+}
+
+// UpdateX (gmres.py:133-136) from the device solution: x += M(sum_i V_i y_i)
+// over the cycle's jend + 1 basis vectors, y rounded to the vectors' dtype
+// as the host path does.
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_gm_update_x_loop(int64_t n, const T *__restrict__ V,
+                                                         const T *__restrict__ diag,
+                                                         T *__restrict__ x,
+                                                         const GmresState *st) {
+  if (st->done) return;
+  const int jc = (int)st->jend + 1;
+  ESR_GRID_ROWS(r, n) {
+    T dx = T(0);
+    for (int i = 0; i < jc; ++i) dx = dx + V[(int64_t)i * n + r] * (T)st->y[i];
     x[r] = x[r] + (diag ? dx / diag[r] : dx);
   }
 }
@@ -965,7 +1082,7 @@ ESR_API int esr_gmres_residual(int dtype, int64_t n, const int32_t *rowptr, cons
   const unsigned g = red_grid(n);
   ESR_FLOAT_DISPATCH(dtype, (k_gm_residual<T><<<g, kRB, 0, s>>>(
                                 n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)b,
-                                (const T *)x, (T *)r, (double *)ws)));
+                                (const T *)x, (T *)r, (double *)ws, nullptr)));
   k_gm_residual_fin<<<1, kRB, 0, s>>>(g, (const double *)ws, (GmresState *)state);
   ESR_LAUNCHED();
   return ESR_OK;
@@ -1023,6 +1140,68 @@ ESR_API int esr_gmres_cycle(int dtype, int64_t n, int64_t restart, const int32_t
   ESR_LAUNCHED();
   return ESR_OK;
   ESR_GUARD_END
+}
+
+// The restart loop on the device: esr_gmres_loop_arm after esr_gmres_begin
+// (tolerance from ||b||, cycle cap, counters); each esr_gmres_loop_cycles call
+// enqueues `count` restart cycles -- true residual and loop test, Arnoldi
+// cycle, least squares, update of x -- every kernel a no-op once the loop
+// has exited (safe to capture and replay; the host reads `done`, `iters`
+// and ||r|| with esr_gmres_loop_read).
+ESR_API int esr_gmres_loop_arm(void *state, double rtol, double atol, int64_t maxiter,
+                               void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(state, "esr_gmres_loop_arm: null state");
+  k_gm_loop_arm<<<1, 1, 0, as_stream(stream)>>>((GmresState *)state, rtol, atol, maxiter);
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_gmres_loop_cycles(int dtype, int64_t n, int64_t restart, const int32_t *rowptr,
+                                  const int32_t *col, const void *Ae, const void *Av,
+                                  const void *diag, const void *b, void *x, void *r, void *V,
+                                  void *w0, void *w1, void *state, void *ws, int64_t count,
+                                  void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1 && restart >= 1 && restart <= kGmresMaxRestart && count >= 0,
+              "esr_gmres_loop_cycles: bad sizes");
+  ESR_REQUIRE(rowptr && col && Ae && Av && b && x && r && V && w0 && w1 && state && ws,
+              "esr_gmres_loop_cycles: null argument");
+  hipStream_t s = as_stream(stream);
+  const unsigned g = red_grid(n);
+  GmresState *st = (GmresState *)state;
+  for (int64_t c = 0; c < count; ++c) {
+    ESR_FLOAT_DISPATCH(dtype, (k_gm_residual<T><<<g, kRB, 0, s>>>(
+                                  n, rowptr, col, (const T *)Ae, (const T *)Av, (const T *)b,
+                                  (const T *)x, (T *)r, (double *)ws, st)));
+    k_gm_residual_fin_loop<<<1, kRB, 0, s>>>(g, (const double *)ws, st);
+    ESR_LAUNCHED();
+    const int rc = esr_gmres_cycle(dtype, n, restart, rowptr, col, Ae, Av, diag, r, V, w0, w1,
+                                   state, ws, stream);
+    if (rc != ESR_OK) return rc;
+    k_gm_lstsq<<<1, 64, 0, s>>>(st);
+    ESR_FLOAT_DISPATCH(dtype, (k_gm_update_x_loop<T><<<g, kRB, 0, s>>>(
+                                  n, (const T *)V, (const T *)diag, (T *)x, st)));
+    ESR_LAUNCHED();
+  }
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+// scalars = {rnorm, bnorm}, ints = {iters, done}.
+ESR_API int esr_gmres_loop_read(const void *state, double *scalars, int64_t *ints,
+                                void *stream) {
+  ESR_REQUIRE(state && scalars && ints, "esr_gmres_loop_read: bad argument");
+  GmresState h;
+  hipStream_t s = as_stream(stream);
+  ESR_HIP(hipMemcpyAsync(&h, state, sizeof(h), hipMemcpyDeviceToHost, s));
+  ESR_HIP(hipStreamSynchronize(s));
+  scalars[0] = h.rnorm;
+  scalars[1] = h.bnorm;
+  ints[0] = h.iters;
+  ints[1] = h.done;
+  return ESR_OK;
 }
 
 ESR_API int esr_gmres_update_x(int dtype, int64_t n, int64_t jcount, const void *V,

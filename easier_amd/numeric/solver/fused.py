@@ -52,6 +52,9 @@ nat.register_signatures({
     'esr_gmres_cycle': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 11 + [_c_vp]),
     'esr_gmres_update_x': (_c_i32, [_c_i32, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     'esr_gmres_read': (_c_i32, [_c_vp, _c_i64, _P_dbl, _P_i64, _P_dbl, _c_vp]),
+    'esr_gmres_loop_arm': (_c_i32, [_c_vp, _c_dbl, _c_dbl, _c_i64, _c_vp]),
+    'esr_gmres_loop_cycles': (_c_i32, [_c_i32, _c_i64, _c_i64] + [_c_vp] * 13 + [_c_i64, _c_vp]),
+    'esr_gmres_loop_read': (_c_i32, [_c_vp, _P_dbl, _P_i64, _c_vp]),
 })
 
 
@@ -350,6 +353,7 @@ class FusedGMRES:
         self.y = torch.zeros(m, dtype=self.op.dtype, device=dev)
         self.dt = nat.dtype_code(self.op.dtype)
         self.graph = None
+        self.loop_graph = None
 
     def _csr(self):
         return [self.op.rowptr.data_ptr(), self.op.col.data_ptr(),
@@ -371,7 +375,52 @@ class FusedGMRES:
                  H.ctypes.data_as(_P_dbl) if with_H else None, nat.stream_ptr())
         return sc[0], sc[1], it[0], it[1], (H.reshape(m + 1, m) if with_H else None)
 
+    # restart cycles per captured graph of the device loop
+    LOOP_CYCLES = 4
+
+    def _loop(self, sp):
+        g = self.gm
+        nat.call('esr_gmres_loop_cycles', self.dt, self.op.n, g.restart, *self._csr(),
+                 None if self.diag is None else self.diag.data_ptr(), g.b.data.data_ptr(),
+                 g.x.data.data_ptr(), g.r.data.data_ptr(), self.V.data_ptr(),
+                 self.w[0].data_ptr(), self.w[1].data_ptr(), self.state.data_ptr(),
+                 self.ws.data_ptr(), self.LOOP_CYCLES, sp)
+
+    def _solve_device(self, rtol, atol, maxiter):
+        """The restart loop on the device: graphs of LOOP_CYCLES cycles replayed
+        until the device reports the exit (no host least squares)."""
+        g = self.gm
+        name = g.__class__.__name__
+        self.bound.check()
+        if self.loop_graph is None:
+            self.loop_graph = _Graph(self._loop)
+        sp = nat.stream_ptr()
+        nat.call('esr_gmres_begin', self.dt, self.op.n, g.b.data.data_ptr(),
+                 self.state.data_ptr(), self.ws.data_ptr(), sp)
+        nat.call('esr_gmres_loop_arm', self.state.data_ptr(), float(rtol),
+                 float(atol) if atol else 0.0, -1 if maxiter is None else int(maxiter), sp)
+        sc = (ctypes.c_double * 2)()
+        it = (ctypes.c_int64 * 2)()
+        while True:
+            self.loop_graph.replay()
+            nat.call('esr_gmres_loop_read', self.state.data_ptr(), sc, it, nat.stream_ptr())
+            if it[1]:
+                break
+        rnorm, bnorm, iters = sc[0], sc[1], int(it[0])
+        _, _, _, _, H = self._read(True)
+        g.easier_host_H[:] = H
+        ops.fill_(g.rnorm.data, rnorm)
+        ops.fill_(g.bnorm.data, bnorm)
+        logger.info(f"{name} solver completed with residual {rnorm}"
+                    f" at the {iters}-th iteration")
+        return {'residual': rnorm, 'iters': iters}
+
     def solve(self, rtol=1e-5, atol=None, maxiter=None, debug_iter=None):
+        """With debug_iter the restart loop runs on the host (residual logged
+        every cycle, least squares by the reference's thin SVD); otherwise on
+        the device."""
+        if debug_iter is None:
+            return self._solve_device(rtol, atol, maxiter)
         from .gmres import lstsq_svd
         g = self.gm
         name = g.__class__.__name__

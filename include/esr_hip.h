@@ -351,6 +351,26 @@ int esr_gmres_cycle(int dtype, int64_t n, int64_t restart,
                     void *w0, void *w1, void *state, void *ws, void *stream);
 int esr_gmres_update_x(int dtype, int64_t n, int64_t jcount, const void *V,
                        const void *y, const void *diag, void *x, void *stream);
+/* The restart loop of gmres.py:214-266 on the device (the host loop above
+ * reads H and solves the least-squares problem on the host every cycle):
+ * esr_gmres_loop_arm after esr_gmres_begin sets tol = max(rtol ||b||, atol)
+ * (atol <= 0: rtol ||b||) and the cycle cap (maxiter < 0: none);
+ * esr_gmres_loop_cycles enqueues `count` restart cycles, each the true
+ * residual r = b - A x with the loop test, the Arnoldi cycle, the
+ * least-squares solve min ||rnorm e_0 - H y|| (Givens rotations; the
+ * reference uses a thin SVD, gmres.py:255-258) and x += M(V y), every kernel
+ * a no-op once the loop has exited (safe to capture and replay);
+ * esr_gmres_loop_read: scalars {rnorm, bnorm}, ints {iters, done}.        */
+int esr_gmres_loop_arm(void *state, double rtol, double atol, int64_t maxiter,
+                       void *stream);
+int esr_gmres_loop_cycles(int dtype, int64_t n, int64_t restart,
+                          const int32_t *rowptr, const int32_t *col,
+                          const void *Ae, const void *Av, const void *diag,
+                          const void *b, void *x, void *r, void *V, void *w0,
+                          void *w1, void *state, void *ws, int64_t count,
+                          void *stream);
+int esr_gmres_loop_read(const void *state, double *scalars, int64_t *ints,
+                        void *stream);
 int esr_gmres_read(const void *state, int64_t restart, double *scalars,
                    int64_t *ints, double *H, void *stream);
 

@@ -120,6 +120,32 @@ def test_gmres_matches_oracle(gpu, m, precond):
     assert info['residual'] < 1e-9 * float(torch.norm(b_o)) * 1.0001
 
 
+@pytest.mark.parametrize('precond,maxiter', [(True, None), (False, 3)])
+def test_gmres_device_loop_matches_host_loop(gpu, precond, maxiter):
+    """The restart loop on the device (Givens least squares, several cycles
+    per graph) and on the host (debug_iter: the reference's thin-SVD least
+    squares every cycle) run the same cycles to the same residual, and a cycle
+    cap stops both after the same number of cycles."""
+    print(f'gmres device vs host loop precond={precond} maxiter={maxiter}', flush=True)
+    out = []
+    for debug in (None, 1000):
+        eqn = Poisson5(60, seed=2)
+        M = Jacobi(eqn.Ac) if precond else None
+        sol = GMRES(eqn.A, eqn.b, eqn.x, M, restart=10)
+        esr.compile([eqn, sol], 'hip')
+        info = sol.solve(rtol=1e-10, maxiter=maxiter, debug_iter=debug)
+        out.append((info, eqn.x.collect().clone(), sol.rnorm.collect().clone()))
+    (a, xa, ra), (b, xb, rb) = out
+    assert a['iters'] == b['iters']
+    if maxiter is not None:
+        assert a['iters'] == maxiter
+    # x differs at rounding level between the two least-squares methods,
+    # which shows in the last digits of residuals near 1e-10 ||b||
+    assert abs(a['residual'] - b['residual']) <= 1e-6 * b['residual']
+    assert float(ra[0]) == a['residual']
+    torch.testing.assert_close(xa, xb, rtol=1e-9, atol=1e-12)
+
+
 def test_reducer_out_and_collect_in_program(gpu):
     class M(esr.Module):
         def __init__(self):
