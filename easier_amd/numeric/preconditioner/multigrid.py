@@ -199,8 +199,8 @@ class MultigridPreconditioner:
             L.cAe = L.Ae if cd == torch.float64 else L.Ae.to(cd)
             L.cAv = L.Av if cd == torch.float64 else L.Av.to(cd)
             L.sell = None
-            if L.Ainv is None:
-                from ..solver.fused import make_sell
+            from ..solver.fused import FusedCG, make_sell
+            if L.Ainv is None and FusedCG.use_sell:
                 L.sell = make_sell(types.SimpleNamespace(rowptr=L.rowptr, col=L.col, Ae=L.cAe,
                                                          Av=L.cAv, n=L.nv))
             L.wd = ops.ewise('div', self.omega, L.Av)         # w / d, float64

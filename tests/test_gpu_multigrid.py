@@ -203,6 +203,32 @@ def test_multigrid_fcg_through_compile(gpu, smoother):
     assert np.linalg.norm(x - x2) <= 1e-6 * np.linalg.norm(x2)
 
 
+@pytest.mark.parametrize('cycle_dtype', [torch.float64, torch.float32])
+def test_multigrid_fcg_sell_and_csr_give_the_same_bits(gpu, cycle_dtype):
+    """The level sweeps and flexible-CG products on the operators' SELL-64
+    form and on their CSR give the same iterates: same steps, same residual,
+    same solution bits."""
+    from easier_amd.numeric.solver import fused
+    print(f'multigrid FCG SELL vs CSR, cycle {cycle_dtype}', flush=True)
+    out = []
+    old = fused.FusedCG.use_sell
+    try:
+        for use in (True, False):
+            fused.FusedCG.use_sell = use
+            M = mg.MultigridPreconditioner(None, omega=0.8, cycle_dtype=cycle_dtype)
+            eqn, sol, _ = _compiled(64, 3, M)
+            info = sol.solve(rtol=1e-10, maxiter=200)
+            assert all((L.sell is not None) == (use and L.Ainv is None) for L in M.levels)
+            assert (sol.easier_kernels.sell is not None) == use
+            out.append((info['iters'], info['residual'], eqn.x.collect()))
+    finally:
+        fused.FusedCG.use_sell = old
+    (ia, ra, xa), (ib, rb, xb) = out
+    print(f'  steps {ia} / {ib}, residual {ra:.6e} / {rb:.6e}', flush=True)
+    assert ia == ib and ra == rb
+    assert torch.equal(xa, xb)
+
+
 def test_float32_cycle_matches_oracle(gpu):
     """The cycle in float32 (float32 copies of the float64 Galerkin
     operators) against the oracle's float64 cycle: float32 rounding only."""
