@@ -574,12 +574,14 @@ def gnn_application_roofline(app):
 
 
 def learned_config5(weights, seeds=(1,), reps=1,
-                    methods=('jacobi', 'mg', 'mg-learned'), skip_cpu=False):
+                    methods=('jacobi', 'mg', 'mg-learned', 'amg'), skip_cpu=False):
     """Learned-preconditioner flexible CG against Jacobi-PCG on unseen
     variable-coefficient problems on the 2048 x 2048 mesh, all to 1e-8:
     'mg-learned' is flexible CG preconditioned by a multigrid V-cycle whose
     smoother is the trained network, 'mg' the same cycle with damped-Jacobi
-    smoothing (the method without the network)."""
+    smoothing (the method without the network), 'amg' the same cycle on a
+    hierarchy built by smoothed aggregation from the operator alone (the
+    form that applies to any mesh)."""
     from easier_amd.numeric.preconditioner import learned as lp
     from easier_amd.workloads import learned as wl
     from easier_amd.workloads import poisson as wp
@@ -771,7 +773,7 @@ def main():
     if '4' in extra:
         result['mixed_config4'] = mixed_config4(skip_cpu=args.skip_cpu)
     if '5' in extra:
-        log('config 5: Jacobi-PCG, multigrid and learned-multigrid flexible CG, 2048^2')
+        log('config 5: Jacobi-PCG, multigrid (geometric, learned, aggregation) flexible CG, 2048^2')
         result['learned_config5'] = learned_config5(args.weights, skip_cpu=args.skip_cpu)
         r5 = result['learned_config5'][0]
         if 'jacobi' in r5 and 'mg-learned' in r5:
@@ -785,6 +787,8 @@ def main():
                 'learned_speedup_vs_jacobi': r5['mg-learned']['speedup_vs_jacobi'],
                 'mg_fcg_s': r5.get('mg', {}).get('seconds'),
                 'mg_fcg_iters': r5.get('mg', {}).get('iters'),
+                'amg_fcg_s': r5.get('amg', {}).get('seconds'),
+                'amg_fcg_iters': r5.get('amg', {}).get('iters'),
             }
     print(json.dumps(result), flush=True)
 

@@ -34,7 +34,7 @@ CXX_FLAGS = ['-O3', '-fPIC', '-std=c++17', '-ffp-contract=off', f'-I{INCLUDE}',
 
 HIP_SOURCES = ['runtime.hip', 'k_basic.hip', 'k_solver.hip', 'k_sell.hip', 'k_batched.hip',
                'k_mrhs.hip', 'k_gnn.hip', 'k_fcg.hip', 'k_mg.hip']
-CXX_SOURCES = ['host_mesh.cpp', 'host_fem.cpp']
+CXX_SOURCES = ['host_mesh.cpp', 'host_fem.cpp', 'host_amg.cpp']
 
 
 def _run(cmd):

@@ -242,13 +242,13 @@ __global__ __launch_bounds__(kMB) void k_mg_jacobi_pre(int64_t n, const Op A,
 // every neighbour (a rowptr -> col -> parent -> zc load chain), which runs
 // about 3x slower on the finest level than the two-pass form.
 template <typename T, typename Op>
-__global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nf, const int32_t *__restrict__ par,
+__global__ __launch_bounds__(kMB) void k_mg_jacobi_post(int64_t nv, const int32_t *__restrict__ par,
                                                        const Op A, const T *__restrict__ wd,
                                                        const T *__restrict__ r,
                                                        const T *__restrict__ z,
                                                        const T *__restrict__ zc,
                                                        T *__restrict__ out) {
-  const int64_t nv = (nf + 1) * (nf + 1), i = mg_tid();
+  const int64_t i = mg_tid();
   if (i >= nv) return;
   if (zc) {
     auto y = [&](int64_t j) { return z[j] + prolong_value<T>(j, par, zc); };
@@ -293,6 +293,73 @@ __global__ __launch_bounds__(kMB) void k_mg_dense(int64_t n, const double *__res
   for (int64_t j = lane; j < n; j += 64) s += Ainv[row * n + j] * (double)r[j];
   s = kr_wave_sum(s);
   if (lane == 0) z[row] = (T)s;
+}
+
+// ---------------------------------------------------------------------------
+// Algebraic levels (smoothed aggregation, set up on the host): the transfers
+// are general CSR operators, P by fine row and R = P^T by coarse row, and
+// the coarse operators general CSR matrices with rows of 10-60 entries.
+// A thread per row would walk such a row as one chain of dependent loads
+// (about 250 us for the 700 k rows of ~20 entries that restrict a 4.2 M-row
+// level), so rows are spread over L lanes (L a power of two <= 64, the L
+// lanes of a row inside one wavefront): lane l folds entries l, l + L, ...
+// in order from 0, and the L partial sums are combined by an xor butterfly,
+// whose sums are bitwise the same on every lane (IEEE addition commutes).
+// ---------------------------------------------------------------------------
+template <int L, typename T, typename Src>
+__device__ __forceinline__ T csr_row_lanes(int32_t b, int32_t e, int lane,
+                                           const int32_t *__restrict__ col,
+                                           const T *__restrict__ val, Src src) {
+  T s = T(0);
+  for (int32_t k = b + lane; k < e; k += L) s = s + val[k] * src(col[k]);
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) s = s + __shfl_xor(s, off, L);
+  return s;
+}
+
+// y = R t (add 0) or z += P zc (add 1), L lanes per row.
+template <typename T, int L>
+__global__ __launch_bounds__(kMB) void k_amg_transfer(int64_t m, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     const T *__restrict__ val,
+                                                     const T *__restrict__ x, T *__restrict__ y,
+                                                     int add) {
+  const int64_t t = mg_tid(), i = t / L;
+  const int lane = (int)(t & (L - 1));
+  if (i >= m) return;
+  const T s = csr_row_lanes<L, T>(rowptr[i], rowptr[i + 1], lane, col, val,
+                                  [&](int64_t j) { return x[j]; });
+  if (lane == 0) y[i] = add ? y[i] + s : s;
+}
+
+// Damped-Jacobi sweeps of a CSR level, L lanes per row; the row product is
+// the Linsys fold (off-diagonal entries, then the diagonal term):
+//   mode 0 (pre):      z = wd r; t = r - A z
+//   mode 1 (post):     out = z + wd (r - A z)
+//   mode 2 (residual): t = r - A z
+template <typename T, int L, int MODE>
+__global__ __launch_bounds__(kMB) void k_amg_sweep(int64_t n, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col,
+                                                  const T *__restrict__ Ae,
+                                                  const T *__restrict__ Av,
+                                                  const T *__restrict__ wd,
+                                                  const T *__restrict__ r, T *__restrict__ z,
+                                                  T *__restrict__ out) {
+  const int64_t t = mg_tid(), i = t / L;
+  const int lane = (int)(t & (L - 1));
+  if (i >= n) return;
+  auto src = [&](int64_t j) -> T { return MODE == 0 ? wd[j] * r[j] : z[j]; };
+  const T s = csr_row_lanes<L, T>(rowptr[i], rowptr[i + 1], lane, col, Ae, src);
+  if (lane != 0) return;
+  const T az = s + src(i) * Av[i];
+  if (MODE == 0) {
+    z[i] = wd[i] * r[i];
+    out[i] = r[i] - az;
+  } else if (MODE == 1) {
+    out[i] = z[i] + wd[i] * (r[i] - az);
+  } else {
+    out[i] = r[i] - az;
+  }
 }
 
 inline unsigned mg_grid(int64_t n) { return (unsigned)ceil_div(n, kMB); }
@@ -399,10 +466,9 @@ int mg_pre(int64_t n, const Op &A, const void *wd, const void *r, void *z, void 
   return ESR_OK;
 }
 template <typename T, typename Op>
-int mg_post(int64_t nf, const int32_t *par, const Op &A, const void *wd, const void *r,
+int mg_post(int64_t nv, const int32_t *par, const Op &A, const void *wd, const void *r,
             const void *z, const void *zc, void *out, hipStream_t s) {
-  const int64_t nv = (nf + 1) * (nf + 1);
-  k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, s>>>(nf, par, A, (const T *)wd, (const T *)r,
+  k_mg_jacobi_post<T><<<mg_grid(nv), kMB, 0, s>>>(nv, par, A, (const T *)wd, (const T *)r,
                                                   (const T *)z, (const T *)zc, (T *)out);
   ESR_LAUNCHED();
   return ESR_OK;
@@ -460,7 +526,8 @@ ESR_API int esr_mg_jacobi_post(int dtype, int64_t nf, const int32_t *par, const 
   const uint8_t *cnt = nullptr;
   const void *colv = nullptr, *val = nullptr;
   const int idx16 = 0;
-  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nf, par, A, wd, r, z, zc, out,
+  const int64_t nv = (nf + 1) * (nf + 1);
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nv, par, A, wd, r, z, zc, out,
                                                          as_stream(stream)))));
   ESR_GUARD_END
 }
@@ -517,7 +584,8 @@ ESR_API int esr_mg_jacobi_post_sell(int dtype, int64_t nf, const int32_t *par,
   ESR_REQUIRE(out != z, "esr_mg_jacobi_post_sell: out must not alias z");
   const int32_t *rowptr = nullptr, *col = nullptr;
   const void *Ae = nullptr;
-  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nf, par, A, wd, r, z, zc, out,
+  const int64_t nv = (nf + 1) * (nf + 1);
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(nv, par, A, wd, r, z, zc, out,
                                                          as_stream(stream)))));
   ESR_GUARD_END
 }
@@ -541,6 +609,100 @@ ESR_API int esr_mg_residual_sell(int dtype, int64_t n, int64_t nf, const int32_t
   const void *Ae = nullptr;
   MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_res<T>(n, nf, par, A, r, z, zc, y, t,
                                                         as_stream(stream)))));
+  ESR_GUARD_END
+}
+
+// Algebraic levels: transfers and sweeps with `lanes` lanes per row
+// (1, 2, 4, ..., 64).
+#define AMG_LANES_DISPATCH(lanes, CALL)                                        \
+  do {                                                                         \
+    switch (lanes) {                                                           \
+    case 1: { constexpr int L = 1; CALL; } break;                              \
+    case 2: { constexpr int L = 2; CALL; } break;                              \
+    case 4: { constexpr int L = 4; CALL; } break;                              \
+    case 8: { constexpr int L = 8; CALL; } break;                              \
+    case 16: { constexpr int L = 16; CALL; } break;                            \
+    case 32: { constexpr int L = 32; CALL; } break;                            \
+    case 64: { constexpr int L = 64; CALL; } break;                            \
+    default: set_error("lanes %d is not a power of two <= 64", (int)(lanes));   \
+      return ESR_ERR_INVALID;                                                  \
+    }                                                                          \
+  } while (0)
+
+namespace {
+inline unsigned amg_grid(int64_t rows, int lanes) { return mg_grid(rows * lanes); }
+}
+
+ESR_API int esr_amg_restrict(int dtype, int64_t nc, const int32_t *r_rowptr, const int32_t *r_col,
+                             const void *r_val, const void *t, void *rc, int32_t lanes,
+                             void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nc >= 1, "esr_amg_restrict: nc must be >= 1");
+  ESR_REQUIRE(r_rowptr && r_col && r_val && t && rc && t != rc,
+              "esr_amg_restrict: null or aliased pointer");
+  MG_DTYPE_DISPATCH(dtype, AMG_LANES_DISPATCH(lanes, (k_amg_transfer<T, L><<<
+      amg_grid(nc, L), kMB, 0, as_stream(stream)>>>(nc, r_rowptr, r_col, (const T *)r_val,
+                                                      (const T *)t, (T *)rc, 0))));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_amg_prolong(int dtype, int64_t nf, const int32_t *p_rowptr, const int32_t *p_col,
+                            const void *p_val, const void *zc, void *z, int32_t lanes,
+                            void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(nf >= 1, "esr_amg_prolong: nf must be >= 1");
+  ESR_REQUIRE(p_rowptr && p_col && p_val && zc && z && zc != z,
+              "esr_amg_prolong: null or aliased pointer");
+  MG_DTYPE_DISPATCH(dtype, AMG_LANES_DISPATCH(lanes, (k_amg_transfer<T, L><<<
+      amg_grid(nf, L), kMB, 0, as_stream(stream)>>>(nf, p_rowptr, p_col, (const T *)p_val,
+                                                      (const T *)zc, (T *)z, 1))));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_amg_sweep(int dtype, int32_t mode, int64_t n, const int32_t *rowptr,
+                          const int32_t *col, const void *Ae, const void *Av, const void *wd,
+                          const void *r, void *z, void *out, int32_t lanes, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_amg_sweep: n must be >= 1");
+  ESR_REQUIRE(mode >= 0 && mode <= 2, "esr_amg_sweep: mode %d is not 0, 1 or 2", (int)mode);
+  ESR_REQUIRE(rowptr && col && Ae && Av && r && z && out && (wd || mode == 2),
+              "esr_amg_sweep: null pointer");
+  ESR_REQUIRE(out != z && out != r, "esr_amg_sweep: out must not alias z or r");
+  hipStream_t s = as_stream(stream);
+  MG_DTYPE_DISPATCH(dtype, AMG_LANES_DISPATCH(lanes, ({
+    const T *ae = (const T *)Ae, *av = (const T *)Av, *w = (const T *)wd, *rr = (const T *)r;
+    if (mode == 0)
+      k_amg_sweep<T, L, 0><<<amg_grid(n, L), kMB, 0, s>>>(n, rowptr, col, ae, av, w, rr,
+                                                         (T *)z, (T *)out);
+    else if (mode == 1)
+      k_amg_sweep<T, L, 1><<<amg_grid(n, L), kMB, 0, s>>>(n, rowptr, col, ae, av, w, rr,
+                                                         (T *)z, (T *)out);
+    else
+      k_amg_sweep<T, L, 2><<<amg_grid(n, L), kMB, 0, s>>>(n, rowptr, col, ae, av, w, rr,
+                                                         (T *)z, (T *)out);
+  })));
+  ESR_LAUNCHED();
+  return ESR_OK;
+  ESR_GUARD_END
+}
+
+ESR_API int esr_amg_jacobi_post_sell(int dtype, int64_t n, const int64_t *soff,
+                                     const uint8_t *cnt, const void *colv, int32_t idx16,
+                                     const void *val, const void *Av, const void *wd,
+                                     const void *r, const void *z, void *out, void *stream) {
+  ESR_GUARD_BEGIN
+  ESR_REQUIRE(n >= 1, "esr_amg_jacobi_post_sell: n must be >= 1");
+  ESR_REQUIRE(soff && cnt && colv && val && Av && wd && r && z && out,
+              "esr_amg_jacobi_post_sell: null pointer");
+  ESR_REQUIRE(out != z, "esr_amg_jacobi_post_sell: out must not alias z");
+  const int32_t *rowptr = nullptr, *col = nullptr;
+  const void *Ae = nullptr;
+  MG_DTYPE_DISPATCH(dtype, MG_OP_DISPATCH(T, (mg_post<T>(n, nullptr, A, wd, r, z, nullptr, out,
+                                                         as_stream(stream)))));
   ESR_GUARD_END
 }
 

@@ -209,10 +209,12 @@ class FcgKernels:
 
 
 class FlexibleCG(esr.Module):
-    """Flexible CG on a compiled Linsys with an optional learned
+    """Flexible CG on a compiled Linsys with an optional learned or multigrid
     preconditioner. `mesh_features` = dict(points (n, 2), logk (n,), n_grid)
-    in the user's vertex order: the network's inputs, renumbered with the
-    program after esr.compile()."""
+    in the user's vertex order: the network's inputs (edge vectors in units
+    of 1/n_grid) and the grid of geometric coarsening, renumbered with the
+    program after esr.compile(). A multigrid preconditioner without a
+    network needs none of them: without a grid it coarsens by aggregation."""
 
     def __init__(self, A: nn.Module, b: esr.Tensor, x: esr.Tensor,
                  M: Optional[LearnedPreconditioner] = None,
@@ -242,23 +244,34 @@ class FlexibleCG(esr.Module):
             from ..preconditioner.multigrid import MultigridPreconditioner, grid_positions
             M = self.easier_M
             f = self.easier_features
-            if f is None:
-                raise RuntimeError('a learned or multigrid preconditioner needs mesh_features')
-            o = self.x.easier_order
-            o = np.arange(op.n) if o is None else o.numpy()
-            points = np.asarray(f['points'])[o]
-            logk = None if f.get('logk') is None else np.asarray(f['logk']).reshape(-1)[o]
-            learned = M.smoother if isinstance(M, MultigridPreconditioner) else M
+            mg = isinstance(M, MultigridPreconditioner)
+            learned = M.smoother if mg else M
+            if f is None and learned is not None:
+                raise RuntimeError('a learned preconditioner needs mesh_features')
+            points = logk = n_grid = None
+            if f is not None:
+                o = self.x.easier_order
+                o = np.arange(op.n) if o is None else o.numpy()
+                points = np.asarray(f['points'])[o]
+                logk = None if f.get('logk') is None else np.asarray(f['logk']).reshape(-1)[o]
+                n_grid = f.get('n_grid')
             graph = None
             if learned is not None:
                 rowptr = self.A.reducer.easier_host_rowptr
                 row = np.repeat(np.arange(op.n), np.diff(rowptr))
                 graph = MeshGraph(row, self.A.selector.easier_host_idx,
                                   op.Av.cpu().numpy(), op.Ae.cpu().numpy(), points, logk,
-                                  f['n_grid'], self.x.device)
-            if isinstance(M, MultigridPreconditioner):
-                M.setup(op.rowptr, op.col, op.Ae, op.Av, f['n_grid'],
-                        grid_positions(points, f['n_grid']), logk, graph)
+                                  n_grid, self.x.device)
+            if mg:
+                # geometric coarsening needs the vertices on the n_grid lattice
+                fpos, grid = None, None
+                if points is not None and n_grid is not None and M.coarsening != 'aggregation':
+                    try:
+                        fpos, grid = grid_positions(points, n_grid), n_grid
+                    except ValueError:
+                        if M.coarsening == 'geometric':
+                            raise
+                M.setup(op.rowptr, op.col, op.Ae, op.Av, grid, fpos, logk, graph)
             else:
                 M.bind(graph)
 

@@ -207,7 +207,8 @@ class TwoGrid:
     def __init__(self, prob: Problem, coarse_cycles=4, omega=0.8):
         from ..numeric.preconditioner.multigrid import MultigridPreconditioner
         coarsest = min(16, max(1, prob.n_grid // 2))
-        self.mg = MultigridPreconditioner(None, omega, coarsest=coarsest).setup(
+        self.mg = MultigridPreconditioner(None, omega, coarsest=coarsest,
+                                         coarsening='geometric').setup(
             prob.rowptr, prob.col, prob.Ae, prob.Av, prob.n_grid)
         if len(self.mg.levels) < 2:
             raise ValueError(f'a {prob.n_grid} x {prob.n_grid} mesh has no coarse level')
@@ -347,20 +348,25 @@ def mean_loss(pre: Optional[LearnedPreconditioner], data, loss='energy', omega=1
     return tot / max(cnt, 1)
 
 
-METHODS = ('jacobi', 'learned', 'mg', 'mg-learned')
+METHODS = ('jacobi', 'learned', 'mg', 'mg-learned', 'amg', 'amg-learned')
 
 
 def make_preconditioner(method, pre: Optional[LearnedPreconditioner], **mg_args):
     """The preconditioner of flexible CG for an evaluation method: 'learned'
     = the network alone, 'mg' = multigrid V-cycle with damped-Jacobi
-    smoothing, 'mg-learned' = the same cycle with the network as smoother."""
+    smoothing (geometric coarsening where it applies), 'mg-learned' = the
+    same cycle with the network as smoother; 'amg' / 'amg-learned' = the
+    same with the hierarchy built by smoothed aggregation (the network on
+    the finest level)."""
     from ..numeric.preconditioner.multigrid import MultigridPreconditioner
     if method == 'learned':
         return pre
-    if method == 'mg':
-        return MultigridPreconditioner(None, **mg_args)
-    if method == 'mg-learned':
-        return MultigridPreconditioner(pre, **mg_args)
+    if method in ('mg', 'amg'):
+        kind = 'aggregation' if method == 'amg' else 'auto'
+        return MultigridPreconditioner(None, coarsening=kind, **mg_args)
+    if method in ('mg-learned', 'amg-learned'):
+        kind = 'aggregation' if method == 'amg-learned' else 'auto'
+        return MultigridPreconditioner(pre, coarsening=kind, **mg_args)
     raise ValueError(f'unknown method {method!r}')
 
 

@@ -565,6 +565,52 @@ int esr_mg_residual_sell(int dtype, int64_t n, int64_t nf, const int32_t *par,
 int esr_mg_dense(int dtype, int64_t n, const double *Ainv, const void *r, void *z,
                  void *stream);
 
+/* ---- algebraic multigrid by smoothed aggregation (new; the same
+ * preconditioner slot, numeric/solver/cg.py:86, for an operator on any mesh)
+ * esr_amg_coarsen (host): one level's set-up from the Linsys form of an SPD
+ *   operator (host arrays; Av > 0): strength |a_ij| >= theta sqrt(a_ii a_jj),
+ *   aggregates in three passes, P = (I - 4/(3 rho) D^-1 A) P_tent with rho
+ *   the Gershgorin bound of D^-1 A (returned), R = P^T and the Galerkin
+ *   operator P^T A P in the Linsys form. *plan owns the result;
+ *   sizes = {nc, nnz(P), off-diagonal nnz of P^T A P}; nc = 0 when no vertex
+ *   has a strong neighbour. nthreads host threads (deterministic results).
+ * esr_amg_fetch (host): copies the plan into caller arrays (any may be
+ *   NULL): agg (n, -1 = not aggregated), P (n+1, nnz(P), nnz(P)),
+ *   R (nc+1, nnz(P), nnz(P)), coarse Linsys (nc+1, nnz, nnz, nc).
+ * esr_amg_destroy (host): frees the plan.
+ * esr_amg_restrict: rc = R t. esr_amg_prolong: z += P zc (CSR transfers,
+ *   values in the dtype).
+ * esr_amg_sweep: damped-Jacobi sweeps of a CSR level (Linsys form): mode 0
+ *   z = wd r, out = r - A z; mode 1 out = z + wd (r - A z); mode 2
+ *   out = r - A z (wd unused, may be NULL).
+ *   These three spread each row over `lanes` lanes (1, 2, 4, ..., 64):
+ *   lane l folds entries l, l + lanes, ... and the partial sums are combined
+ *   by an xor butterfly (deterministic; lanes = 1 is the plain row fold).
+ * esr_amg_jacobi_post_sell: mode 1 on the level's SELL-64 form (one row per
+ *   thread, the fold of esr_mg_jacobi_post).                                */
+int esr_amg_coarsen(int64_t n, const int32_t *rowptr, const int32_t *col,
+                    const double *Ae, const double *Av, double theta, int32_t nthreads,
+                    void **plan, int64_t *sizes, double *rho);
+int esr_amg_fetch(void *plan, int32_t *agg, int32_t *p_rowptr, int32_t *p_col,
+                  double *p_val, int32_t *r_rowptr, int32_t *r_col, double *r_val,
+                  int32_t *c_rowptr, int32_t *c_col, double *c_Ae, double *c_Av);
+int esr_amg_destroy(void *plan);
+int esr_amg_restrict(int dtype, int64_t nc, const int32_t *r_rowptr,
+                     const int32_t *r_col, const void *r_val, const void *t,
+                     void *rc, int32_t lanes, void *stream);
+int esr_amg_prolong(int dtype, int64_t nf, const int32_t *p_rowptr,
+                    const int32_t *p_col, const void *p_val, const void *zc,
+                    void *z, int32_t lanes, void *stream);
+int esr_amg_sweep(int dtype, int32_t mode, int64_t n, const int32_t *rowptr,
+                  const int32_t *col, const void *Ae, const void *Av,
+                  const void *wd, const void *r, void *z, void *out,
+                  int32_t lanes, void *stream);
+int esr_amg_jacobi_post_sell(int dtype, int64_t n, const int64_t *soff,
+                             const uint8_t *cnt, const void *colv, int32_t idx16,
+                             const void *val, const void *Av, const void *wd,
+                             const void *r, const void *z, void *out,
+                             void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,6 @@
 // Drives every host entry point of libesrhip.so (mesh generators, stable CSR
-// build, RCM, block order, bandwidth, P1 assembly) over small and odd sizes,
+// build, RCM, block order, bandwidth, P1 assembly, smoothed-aggregation
+// levels) over small and odd sizes,
 // including the error paths, for a build with AddressSanitizer and
 // UndefinedBehaviorSanitizer (tests/test_sanitize.py). Exit 0 when every
 // call behaved; the sanitizers abort on the first memory or UB error.
@@ -69,6 +70,43 @@ static int mesh_case(int kind, int64_t n, double jitter) {
   CHECK(esr_p1_assemble(np_, nc, cells.data(), points.data(), nullptr, nullptr, nnz,
                         row.data(), col.data(), Ae.data(), Av.data(), nullptr,
                         bnd.data()) == ESR_OK);
+  // smoothed-aggregation levels down to a few rows, on 1 and 3 host threads
+  std::vector<int32_t> rp(np_ + 1, 0), cl(nnz);
+  for (int64_t e = 0; e < nnz; ++e) {
+    rp[row[e] + 1]++;
+    cl[e] = (int32_t)col[e];
+  }
+  for (int64_t v = 0; v < np_; ++v) rp[v + 1] += rp[v];
+  std::vector<double> ae(Ae), av(Av);
+  for (int level = 0; level < 8; ++level) {
+    const int64_t m = (int64_t)av.size();
+    std::vector<int32_t> agg(m);
+    for (int nt : {1, 3}) {
+      void *plan = nullptr;
+      int64_t sz[3];
+      double rho = 0.0;
+      CHECK(esr_amg_coarsen(m, rp.data(), cl.data(), ae.data(), av.data(), 0.08, nt, &plan, sz,
+                            &rho) == ESR_OK);
+      CHECK(plan && rho >= 1.0);
+      const int64_t c = sz[0];
+      std::vector<int32_t> prow(m + 1), pcol(sz[1]), rrow(c + 1), rcol(sz[1]), crow(c + 1),
+          ccol(sz[2]);
+      std::vector<double> pval(sz[1]), rval(sz[1]), cae(sz[2]), cav(c);
+      CHECK(esr_amg_fetch(plan, agg.data(), prow.data(), pcol.data(), pval.data(), rrow.data(),
+                          rcol.data(), rval.data(), crow.data(), ccol.data(), cae.data(),
+                          cav.data()) == ESR_OK);
+      CHECK(esr_amg_fetch(plan, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                          nullptr, nullptr, nullptr, nullptr) == ESR_OK);
+      CHECK(esr_amg_destroy(plan) == ESR_OK);
+      if (nt == 3) {
+        if (c == 0 || c >= m) return 0;
+        rp.swap(crow);
+        cl.swap(ccol);
+        ae.swap(cae);
+        av.swap(cav);
+      }
+    }
+  }
   return 0;
 }
 
@@ -87,6 +125,21 @@ int main() {
   int64_t s0, s1, s2, s3;
   CHECK(esr_mesh_sizes(3, 4, &s0, &s1, &s2, &s3) != ESR_OK);
   CHECK(esr_mesh_sizes(2, 0, &s0, &s1, &s2, &s3) != ESR_OK);
+  // a zero diagonal, a column out of range and an indefinite operator
+  std::vector<int32_t> rp2 = {0, 1, 2}, cl2 = {1, 0}, cl3 = {1, 2};
+  std::vector<double> ae2 = {-1.0, -1.0}, av2 = {2.0, 0.0}, av3 = {1.0, 1.0},
+                      ae3 = {-4.0, -4.0};
+  void *plan = nullptr;
+  int64_t sz[3];
+  double rho;
+  CHECK(esr_amg_coarsen(2, rp2.data(), cl2.data(), ae2.data(), av2.data(), 0.08, 1, &plan, sz,
+                        &rho) == ESR_ERR_INVALID && plan == nullptr);
+  CHECK(esr_amg_coarsen(2, rp2.data(), cl3.data(), ae2.data(), av3.data(), 0.08, 1, &plan, sz,
+                        &rho) == ESR_ERR_INVALID && plan == nullptr);
+  CHECK(esr_amg_coarsen(2, rp2.data(), cl2.data(), ae3.data(), av3.data(), 0.08, 1, &plan, sz,
+                        &rho) == ESR_ERR_INVALID && plan == nullptr);
+  CHECK(esr_amg_fetch(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, nullptr) == ESR_ERR_INVALID);
   std::printf("host entry points clean\n");
   return 0;
 }

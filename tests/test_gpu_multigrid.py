@@ -294,6 +294,9 @@ def test_setup_rejects_meshes_it_cannot_coarsen(gpu):
     n = 2 * 37                   # 74 -> 37: odd and above 32
     A, fpos, csr, _ = _problem(n, 1, False)
     with pytest.raises(ValueError, match='coarsest'):
-        mg.MultigridPreconditioner(coarsest=16).setup(*csr, n, fpos)
+        mg.MultigridPreconditioner(coarsest=16, coarsening='geometric').setup(*csr, n, fpos)
     with pytest.raises(ValueError, match='grid'):
-        mg.MultigridPreconditioner().setup(*csr, n + 1, fpos)
+        mg.MultigridPreconditioner(coarsening='geometric').setup(*csr, n + 1, fpos)
+    # the default coarsens such operators by aggregation instead
+    M = mg.MultigridPreconditioner(coarsest=16).setup(*csr, n, fpos)
+    assert M.kind == 'aggregation' and M.levels[-1].nv <= 1024

@@ -1,5 +1,5 @@
 """The host C++ of libesrhip.so (mesh generators, stable CSR build, RCM,
-block order, bandwidth, P1 assembly) built with AddressSanitizer and
+block order, bandwidth, P1 assembly, smoothed-aggregation levels) built with AddressSanitizer and
 UndefinedBehaviorSanitizer (g++, host code only) and driven over small and
 odd sizes and the error paths by tests/sanitize/host_driver.cpp; any memory
 error or undefined behaviour aborts the driver."""
@@ -21,6 +21,7 @@ def test_host_code_is_clean_under_asan_and_ubsan(tmp_path):
            f'-I{REPO}/include', f'-I{CSRC}',
            os.path.join(REPO, 'tests', 'sanitize', 'host_driver.cpp'),
            os.path.join(CSRC, 'host_mesh.cpp'), os.path.join(CSRC, 'host_fem.cpp'),
+           os.path.join(CSRC, 'host_amg.cpp'),
            '-o', exe]
     build = subprocess.run(cmd, capture_output=True, text=True)
     assert build.returncode == 0, build.stderr
