@@ -163,3 +163,33 @@ def test_float32_aggregation_cycle_reaches_float64_tolerance(gpu):
         _check(_scipy(s), s['b'], eqn.x.collect().numpy(), info, rtol=1e-10, steps=40)
         its[cd] = info['iters']
     assert its[torch.float32] <= its[torch.float64] + 3, its
+
+
+def test_config3_family_with_the_trained_smoother(gpu):
+    """The learned preconditioner on the batched configuration's problem
+    family: sides n drawn uniformly from [32, 512] (the batched benchmark's
+    draw), interior vertices jittered by up to 0.2 h, a seeded log-kappa
+    field. Most such n are not m 2^k with m <= 32, so the multigrid takes
+    the aggregation hierarchy there; with and without the trained network
+    flexible CG reaches 1e-8 on the true residual in a few dozen steps."""
+    import os
+    from easier_amd.numeric.preconditioner import learned as lp
+    from easier_amd.workloads import learned as wl
+    S = lp.load(os.path.join(os.path.dirname(wl.__file__), 'gnn_smoother_l3_h32.npz'))
+    rng = np.random.default_rng(0)
+    sides = [int(rng.integers(32, 513)) for _ in range(4)]
+    kinds = set()
+    for k, n in enumerate(sides):
+        m, s, _ = wp.varcoef_system(n, 100 + k, jitter=0.2)
+        logk = wp.gaussian_random_field(n, 100 + k).reshape(-1)
+        for smoother in (None, S):
+            print(f'config-3 family n={n} smoother={smoother is not None}', flush=True)
+            eqn = wp.PoissonP1(s)
+            M = mg.MultigridPreconditioner(smoother)
+            info = _solve(eqn, M, dict(points=m['points'], logk=logk, n_grid=n))
+            geometric = M.geometric_unfit(s['Av'].size, n) is None
+            assert M.kind == ('geometric' if geometric else 'aggregation')
+            kinds.add(M.kind)
+            print(f'  {M.kind}, {info["iters"]} steps', flush=True)
+            _check(_scipy(s), s['b'], eqn.x.collect().numpy(), info, steps=40)
+    assert 'aggregation' in kinds
