@@ -363,19 +363,19 @@ class MultigridPreconditioner:
         while levels[-1].nv > self.coarse_rows and len(levels) < self.max_levels:
             L = levels[-1]
             lv = amg_coarsen(*host, theta=self.theta * 0.5 ** (len(levels) - 1))
+            L.rho = lv['rho']
             if lv['nc'] == 0 or lv['nc'] > 0.9 * L.nv:
                 break
-            L.rho = lv['rho']
             L.P = tuple(f(a) for f, a in zip((i32, i32, f64), lv['P']))
             L.R = tuple(f(a) for f, a in zip((i32, i32, f64), lv['R']))
             crow, ccol, cAe, cAv = lv['A']
             levels.append(Level(None, i32(crow), i32(ccol), f64(cAe), f64(cAv), nv=lv['nc']))
             host = lv['A']
         last = levels[-1]
-        if last.nv > 8 * self.coarse_rows:
-            raise ValueError(f'aggregation stalled at {last.nv} rows (the operator has too '
-                             'few strong couplings to coarsen)')
-        last.Ainv = self._dense_inverse(*host, dev)
+        if last.nv <= 8 * self.coarse_rows:
+            last.Ainv = self._dense_inverse(*host, dev)
+        # else: too few strong couplings to coarsen further and too large to
+        # invert -- the last level is smoothed only (two damped-Jacobi sweeps)
         self.levels = levels
 
     @staticmethod
@@ -502,6 +502,14 @@ class MultigridPreconditioner:
             nat.call('esr_mg_dense', DT, L.nv, L.Ainv.data_ptr(), r.data_ptr(),
                      out.data_ptr(), sp)
             return
+        if k + 1 == len(self.levels):
+            # an algebraic level that could not be coarsened: smoothing only
+            if L.graph is not None and self.learned_pre:
+                self.smoother.forward_into(L.graph, L.bufs, r, L.z)
+            else:
+                self._pre(L, r, L.z, L.t)
+            self._post_sweep(L, r, out)
+            return
         C = self.levels[k + 1]
         if L.graph is None:
             self._pre(L, r, L.z, L.t)
@@ -525,17 +533,21 @@ class MultigridPreconditioner:
         else:
             self._post(L, C, r, out)
 
+    def _post_sweep(self, L, r, out):
+        """out = z + w (r - A z) / d on an algebraic level."""
+        if L.sell is not None:
+            nat.call('esr_amg_jacobi_post_sell', self.dt, L.nv, *L.sell.arrays(),
+                     L.cAv.data_ptr(), L.wd.data_ptr(), r.data_ptr(), L.z.data_ptr(),
+                     out.data_ptr(), nat.stream_ptr())
+        else:
+            self._sweep(L, 1, r, L.z, out)
+
     def _post(self, L, C, r, out):
         """z += P zc (in place), then out = z + w (r - A z) / d."""
         sp, DT = nat.stream_ptr(), self.dt
         self._prolong(L, C, L.z)
         if L.P is not None:
-            if L.sell is not None:
-                nat.call('esr_amg_jacobi_post_sell', DT, L.nv, *L.sell.arrays(),
-                         L.cAv.data_ptr(), L.wd.data_ptr(), r.data_ptr(), L.z.data_ptr(),
-                         out.data_ptr(), sp)
-            else:
-                self._sweep(L, 1, r, L.z, out)
+            self._post_sweep(L, r, out)
         elif L.sell is not None:
             nat.call('esr_mg_jacobi_post_sell', DT, L.n_grid, None, *L.sell.arrays(),
                      L.cAv.data_ptr(), L.wd.data_ptr(), r.data_ptr(), L.z.data_ptr(), None,

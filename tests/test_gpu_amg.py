@@ -193,3 +193,29 @@ def test_config3_family_with_the_trained_smoother(gpu):
             print(f'  {M.kind}, {info["iters"]} steps', flush=True)
             _check(_scipy(s), s['b'], eqn.x.collect().numpy(), info, steps=40)
     assert 'aggregation' in kinds
+
+
+def test_operator_without_strong_couplings_is_smoothed_only(gpu):
+    """theta above every coupling: nothing aggregates, the level is too
+    large for the dense solve, and the cycle is two damped-Jacobi sweeps
+    (weight 0.8 * 2 / rho); flexible CG still converges."""
+    print('AMG smoother-only level tri2 n=37', flush=True)
+    _, s, _ = wp.varcoef_system(37, 15, jitter=0.2)
+    A = _scipy(s)
+    M = mg.MultigridPreconditioner(None, coarsening='aggregation', theta=0.999,
+                                   coarse_rows=64).setup(*_device_csr(s, gpu))
+    assert len(M.levels) == 1 and M.levels[0].Ainv is None
+    d = A.diagonal()
+    rho = np.max(np.abs(A).sum(axis=1).A1 / d)
+    w = 0.8 * 2.0 / rho
+    r = np.random.default_rng(3).standard_normal(A.shape[0])
+    z = M(torch.from_numpy(r).to(gpu)).cpu().numpy()
+    z0 = w * r / d
+    want = z0 + w * (r - A @ z0) / d
+    assert np.abs(z - want).max() <= 1e-12 * np.abs(want).max()
+    eqn = wp.PoissonP1(s)
+    sol = FlexibleCG(eqn.A, eqn.b, eqn.x, mg.MultigridPreconditioner(
+        None, coarsening='aggregation', theta=0.999, coarse_rows=64))
+    esr.compile([eqn, sol], 'hip')
+    info = sol.solve(rtol=1e-8, maxiter=2000)
+    _check(A, s['b'], eqn.x.collect().numpy(), info, steps=2000)
