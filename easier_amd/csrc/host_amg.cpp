@@ -28,7 +28,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <exception>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -63,15 +65,26 @@ void run_rows(int64_t nrows, int nthreads, std::vector<RowChunks> &out, RowFn ma
   const int64_t nchunks = (nrows + chunk - 1) / chunk;
   out.assign((size_t)nchunks, RowChunks());
   std::atomic<int64_t> next(0);
+  std::atomic<bool> failed(false);
+  std::exception_ptr error;
+  std::mutex error_lock;
+  // a worker's exception (e.g. std::bad_alloc) is kept and rethrown on the
+  // calling thread after every worker has joined
   auto work = [&]() {
-    auto row = make_worker();   // per-worker scratch (sparse accumulator)
-    for (;;) {
-      const int64_t c = next.fetch_add(1);
-      if (c >= nchunks) break;
-      RowChunks &o = out[(size_t)c];
-      const int64_t lo = c * chunk, hi = std::min(nrows, lo + chunk);
-      o.len.reserve((size_t)(hi - lo));
-      for (int64_t r = lo; r < hi; ++r) row(r, o);
+    try {
+      auto row = make_worker();   // per-worker scratch (sparse accumulator)
+      for (;;) {
+        const int64_t c = next.fetch_add(1);
+        if (c >= nchunks || failed.load()) break;
+        RowChunks &o = out[(size_t)c];
+        const int64_t lo = c * chunk, hi = std::min(nrows, lo + chunk);
+        o.len.reserve((size_t)(hi - lo));
+        for (int64_t r = lo; r < hi; ++r) row(r, o);
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> g(error_lock);
+      if (!error) error = std::current_exception();
+      failed.store(true);
     }
   };
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, nchunks));
@@ -82,13 +95,19 @@ void run_rows(int64_t nrows, int nthreads, std::vector<RowChunks> &out, RowFn ma
     for (int t = 0; t < nt; ++t) th.emplace_back(work);
     for (auto &t : th) t.join();
   }
+  if (error) std::rethrow_exception(error);
 }
 
-// Concatenate chunked rows into a CSR (rowptr of nrows + 1).
-void gather_rows(const std::vector<RowChunks> &parts, int64_t nrows, std::vector<int32_t> &rowptr,
+// Concatenate chunked rows into a CSR (rowptr of nrows + 1). False when the
+// entries do not fit int32 row pointers.
+bool gather_rows(const std::vector<RowChunks> &parts, int64_t nrows, std::vector<int32_t> &rowptr,
                  std::vector<int32_t> &col, std::vector<double> &val) {
+  int64_t tot = 0;
+  for (const auto &p : parts) tot += (int64_t)p.col.size();
+  if (tot > (int64_t)INT32_MAX) return false;
   rowptr.assign((size_t)nrows + 1, 0);
-  int64_t r = 0, tot = 0;
+  int64_t r = 0;
+  tot = 0;
   for (const auto &p : parts) {
     for (int32_t l : p.len) {
       tot += l;
@@ -103,6 +122,7 @@ void gather_rows(const std::vector<RowChunks> &parts, int64_t nrows, std::vector
     col.insert(col.end(), p.col.begin(), p.col.end());
     val.insert(val.end(), p.val.begin(), p.val.end());
   }
+  return true;
 }
 
 // Sparse accumulator over [0, m): values summed in arrival order, touched
@@ -213,7 +233,10 @@ int coarsen(AmgPlan &pl, int64_t n, const int32_t *rowptr, const int32_t *col, c
       }
     };
   });
-  gather_rows(parts, n, pl.prow, pl.pcol, pl.pval);
+  if (!gather_rows(parts, n, pl.prow, pl.pcol, pl.pval)) {
+    esr::set_error("esr_amg_coarsen: the prolongator has more than 2^31 - 1 entries");
+    return ESR_ERR_UNSUPPORTED;
+  }
   // R = P^T by a counting sort over fine rows (fine indices ascending per row)
   pl.rrow.assign((size_t)na + 1, 0);
   for (int32_t k : pl.pcol) pl.rrow[(size_t)k + 1]++;
@@ -270,7 +293,10 @@ int coarsen(AmgPlan &pl, int64_t n, const int32_t *rowptr, const int32_t *col, c
       o.len.push_back(len);
     };
   });
-  gather_rows(cparts, na, pl.crow, pl.ccol, pl.cAe);
+  if (!gather_rows(cparts, na, pl.crow, pl.ccol, pl.cAe)) {
+    esr::set_error("esr_amg_coarsen: the coarse operator has more than 2^31 - 1 entries");
+    return ESR_ERR_UNSUPPORTED;
+  }
   pl.cAv.swap(diag);
   for (int64_t I = 0; I < na; ++I) {
     if (!(pl.cAv[(size_t)I] > 0.0)) {
