@@ -20,7 +20,9 @@ namespace esr {
 
 struct MrhsState {  // per right-hand side (same layout as the batched solver's)
   double rz_cur, rz_next, pAp, alpha, rr, rz_new, rnorm, bnorm, tol, rtol, atol, beta;
-  int64_t k_cur, k_next, maxiter, done, iters, pad2;
+  // xpend: this step's x += alpha p is still to be applied (set by k_m_alpha
+  // while the right-hand side runs, cleared by it once the side is done)
+  int64_t k_cur, k_next, maxiter, done, iters, xpend;
 };
 
 namespace {
@@ -116,13 +118,14 @@ __global__ __launch_bounds__(kRB) void k_m_init_fin(int K, int64_t g, const doub
     s->k_cur = s->k_next = 0;
     s->done = 0;
     s->iters = 0;
+    s->xpend = 0;
     if (k == 0) *active = K;
   }
 }
 
 // Step part 1 (cg.py:44-47): AP = A P; partials of p.Ap. On 64 right-hand
 // sides of the 1024 x 1024 mesh this kernel took 0.91 ms of a 1.9 ms step
-// on the 512-workgroup grid of the other kernels (1.3 TB/s on its 1.16 GB:
+// on a 512-workgroup grid (1.3 TB/s on its 1.16 GB:
 // a wavefront serves one row, 513 rows in turn); on its own grid of up to
 // 8,192 workgroups the solve takes 6.03 s instead of 6.48 s (2,048: 6.21 s,
 // 32,768: 6.12 s). It fetches 2.2 GB through the L2s for 0.63 GB of
@@ -156,7 +159,10 @@ __global__ __launch_bounds__(kRB) void k_m_alpha(int K, int64_t g, const double 
                                                 MrhsState *st) {
   const int k = blockIdx.x;
   MrhsState *s = st + k;
-  if (s->done) return;
+  if (s->done) {
+    if (threadIdx.x == 0) s->xpend = 0;
+    return;
+  }
   double tot[1];
   sum_partials<1>(part, 0, (int64_t)k * g, g, tot);
   if (threadIdx.x == 0) {
@@ -164,14 +170,16 @@ __global__ __launch_bounds__(kRB) void k_m_alpha(int K, int64_t g, const double 
     s->rz_cur = s->rz_next;
     s->pAp = tot[0];
     s->alpha = s->rz_cur / tot[0];
+    s->xpend = 1;
   }
 }
 
-// Step part 2 (cg.py:47-52) + first half of Update: x, r, z; partials of
-// r.r and r.z.
+// Step part 2 (cg.py:47-52) + first half of Update: r and the partials of
+// r.r and r.z with z = M r. x is updated by k_m_p (which reads p anyway) and
+// z is recomputed there instead of stored: three vector passes fewer per
+// step, the same values (the same operations on the same operands).
 __global__ __launch_bounds__(kRB) void k_m_xr(int64_t n, int K, const double *__restrict__ diag,
-                                             double *__restrict__ X, double *__restrict__ R,
-                                             double *__restrict__ Z, const double *__restrict__ P,
+                                             double *__restrict__ R,
                                              const double *__restrict__ AP, const MrhsState *st,
                                              double *__restrict__ part) {
   const Lanes ln = lanes_of(K);
@@ -183,10 +191,8 @@ __global__ __launch_bounds__(kRB) void k_m_xr(int64_t n, int K, const double *__
     if (!on) continue;
     const int64_t o = i * K + ln.k;
     const double ri = R[o] - alpha * AP[o];
-    X[o] = X[o] + alpha * P[o];
     R[o] = ri;
     const double zi = diag ? ri / diag[i] : ri;
-    Z[o] = zi;
     acc[0] += ri * ri;
     acc[1] += ri * zi;
   }
@@ -221,17 +227,27 @@ __global__ __launch_bounds__(kRB) void k_m_test(int K, int64_t g, const double *
   s->rz_next = rz[0];
 }
 
-// Update (cg.py:67-71): p = z + beta p.
-__global__ __launch_bounds__(kRB) void k_m_p(int64_t n, int K, const double *__restrict__ Z,
+// The step's x += alpha p (cg.py:46, for every side that ran this step,
+// including one the loop test has just stopped) and Update (cg.py:67-71):
+// p = z + beta p with z = M r, for the sides still running.
+__global__ __launch_bounds__(kRB) void k_m_p(int64_t n, int K, const double *__restrict__ diag,
+                                            double *__restrict__ X, const double *__restrict__ R,
                                             double *__restrict__ P, const MrhsState *st) {
   const Lanes ln = lanes_of(K);
   const MrhsState *s = st + (ln.k < K ? ln.k : 0);
-  const bool on = ln.k < K && !s->done;
-  const double beta = s->beta;
+  const bool xon = ln.k < K && s->xpend;
+  const bool pon = xon && !s->done;
+  const double alpha = s->alpha, beta = s->beta;
   ESR_MRHS_ROWS(i, n, ln) {
-    if (!on) continue;
+    if (!xon) continue;
     const int64_t o = i * K + ln.k;
-    P[o] = Z[o] + beta * P[o];
+    const double pi = P[o];
+    X[o] = X[o] + alpha * pi;
+    if (pon) {
+      const double ri = R[o];
+      const double zi = diag ? ri / diag[i] : ri;
+      P[o] = zi + beta * pi;
+    }
   }
 }
 
@@ -242,10 +258,13 @@ inline unsigned mrhs_grid_cap(int64_t n, int K, int64_t cap) {
   return (unsigned)(g < cap ? (g > 0 ? g : 1) : cap);
 }
 inline unsigned mrhs_grid(int64_t n, int K) { return mrhs_grid_cap(n, K, kMaxGridM); }
-// The product's grid: a wavefront serves one row at a time through dependent
-// row-pointer -> column -> neighbour-row loads, so it needs many wavefronts in
-// flight; its partials (one per workgroup and right-hand side) are summed by
-// k_m_alpha in a fixed order whatever their number.
+// The grid of the step kernels: the product's wavefront serves one row at a
+// time through dependent row-pointer -> column -> neighbour-row loads, and
+// the update kernels stream 3-5 vectors of n K values, so all of them need
+// many wavefronts in flight; their partials (per workgroup and right-hand
+// side) are summed by k_m_alpha / k_m_test in a fixed order whatever their
+// number. (The update kernels on the 512-workgroup grid of k_m_init: 64
+// right-hand sides on 1024^2 in 6.03 s; on this grid 5.47 s.)
 inline unsigned mrhs_spmv_grid(int64_t n, int K) { return mrhs_grid_cap(n, K, kMaxGridSpmvM); }
 
 } // namespace
@@ -258,7 +277,7 @@ ESR_API int esr_mrhs_sizes(int64_t n, int64_t K, int64_t *state_bytes, int64_t *
               "esr_mrhs_sizes: need n >= 1 and 1 <= K <= %d", kMaxK);
   *state_bytes = K * (int64_t)sizeof(MrhsState);
   const int64_t g = mrhs_grid(n, (int)K), g1 = mrhs_spmv_grid(n, (int)K);
-  *ws_bytes = K * (2 * g > g1 ? 2 * g : g1) * (int64_t)sizeof(double);
+  *ws_bytes = K * (2 * g > 2 * g1 ? 2 * g : 2 * g1) * (int64_t)sizeof(double);
   return ESR_OK;
 }
 
@@ -296,9 +315,9 @@ ESR_API int esr_mrhs_steps(int64_t n, int64_t K, const int32_t *rowptr, const in
   for (int64_t it = 0; it < count; ++it) {
     k_m_spmv<<<g1, kRB, 0, s>>>(n, (int)K, rowptr, col, Ae, Av, P, AP, st, ws);
     k_m_alpha<<<(unsigned)K, kRB, 0, s>>>((int)K, g1, ws, st);
-    k_m_xr<<<g, kRB, 0, s>>>(n, (int)K, diag, X, R, Z, P, AP, st, ws);
-    k_m_test<<<(unsigned)K, kRB, 0, s>>>((int)K, g, ws, st, active);
-    k_m_p<<<g, kRB, 0, s>>>(n, (int)K, Z, P, st);
+    k_m_xr<<<g1, kRB, 0, s>>>(n, (int)K, diag, R, AP, st, ws);
+    k_m_test<<<(unsigned)K, kRB, 0, s>>>((int)K, g1, ws, st, active);
+    k_m_p<<<g1, kRB, 0, s>>>(n, (int)K, diag, X, R, P, st);
     ESR_LAUNCHED();
   }
   return ESR_OK;
