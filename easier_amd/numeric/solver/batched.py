@@ -32,7 +32,10 @@ nat.register_signatures({
                                 ctypes.POINTER(_c_i64)]),
     'esr_mrhs_init': (_c_i32, [_c_i64, _c_i64] + [_c_vp] * 14),
     'esr_mrhs_steps': (_c_i32, [_c_i64, _c_i64] + [_c_vp] * 13 + [_c_i64, _c_vp]),
+    'esr_mrhs_ell': (_c_i32, [_c_i64] + [_c_vp] * 7),
+    'esr_mrhs_steps_ell': (_c_i32, [_c_i64, _c_i64] + [_c_vp] * 13 + [_c_i64, _c_vp]),
 })
+ELL_W = 8            # ESR_MRHS_ELL_W
 
 _STATE_DTYPE = np.dtype([
     ('rz_cur', 'f8'), ('rz_next', 'f8'), ('pAp', 'f8'), ('alpha', 'f8'),
@@ -61,6 +64,7 @@ class MultiRhsCG:
     iteration count (cg.py:106-137 per right-hand side)."""
 
     MAX_K = 64
+    use_ell = True       # the padded ELL product when every row fits ELL_W entries
 
     def __init__(self, system: Dict[str, np.ndarray], B: np.ndarray, jacobi=True,
                  device=None):
@@ -78,6 +82,14 @@ class MultiRhsCG:
         self.rowptr, self.col = t32(rowptr), t32(system['col'])
         self.Ae, self.Av = f64(system['Ae']), f64(system['Av'])
         self.diag = self.Av if jacobi else None
+        self.ell = None
+        deg = np.diff(rowptr)
+        if self.use_ell and deg.size and int(deg.max()) <= ELL_W:
+            self.ell = (torch.empty(n, dtype=torch.uint8, device=dev),
+                        torch.empty(n * ELL_W, dtype=torch.int32, device=dev),
+                        torch.empty(n * ELL_W, dtype=torch.float64, device=dev))
+            nat.call('esr_mrhs_ell', n, self.rowptr.data_ptr(), self.col.data_ptr(),
+                     self.Ae.data_ptr(), *[t.data_ptr() for t in self.ell], nat.stream_ptr())
         self.B = f64(B)
         mk = lambda: torch.zeros((n, self.K), dtype=torch.float64, device=dev)
         self.X, self.R, self.Z, self.P, self.AP = mk(), mk(), mk(), mk(), mk()
@@ -95,9 +107,15 @@ class MultiRhsCG:
                 self.Av.data_ptr(), None if self.diag is None else self.diag.data_ptr()]
 
     def _steps(self, count, sp):
-        nat.call('esr_mrhs_steps', self.n, self.K, *self._op(), self.X.data_ptr(),
-                 self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(), self.AP.data_ptr(),
-                 self.state.data_ptr(), self.ws.data_ptr(), self.active.data_ptr(), count, sp)
+        vecs = [self.X.data_ptr(), self.R.data_ptr(), self.Z.data_ptr(), self.P.data_ptr(),
+                self.AP.data_ptr(), self.state.data_ptr(), self.ws.data_ptr(),
+                self.active.data_ptr(), count, sp]
+        if self.ell is not None:
+            nat.call('esr_mrhs_steps_ell', self.n, self.K, *[t.data_ptr() for t in self.ell],
+                     self.Av.data_ptr(), None if self.diag is None else self.diag.data_ptr(),
+                     *vecs)
+        else:
+            nat.call('esr_mrhs_steps', self.n, self.K, *self._op(), *vecs)
 
     def step_time(self, iters=5):
         """Milliseconds per iteration, from HIP events around `iters`

@@ -425,6 +425,22 @@ int esr_mrhs_steps(int64_t n, int64_t K, const int32_t *rowptr, const int32_t *c
                    const double *Ae, const double *Av, const double *diag, double *X,
                    double *R, double *Z, double *P, double *AP, void *state, double *ws,
                    int32_t *active, int64_t count, void *stream);
+/* The same steps on a padded row-major ELL copy of the operator's
+ * off-diagonal part, for operators whose rows hold at most ESR_MRHS_ELL_W
+ * entries (the caller checks): len (n, uint8), ecol (n * W, int32; padding
+ * = the row's own index), eval (n * W, float64; padding 0). esr_mrhs_ell
+ * builds it from the CSR. Bit-identical to esr_mrhs_steps (same folds, same
+ * order); the row's entries are loaded without waiting on row pointers, and
+ * the next row's while this row's neighbours are gathered.               */
+#define ESR_MRHS_ELL_W 8
+int esr_mrhs_ell(int64_t n, const int32_t *rowptr, const int32_t *col,
+                 const double *Ae, uint8_t *len, int32_t *ecol, double *eval,
+                 void *stream);
+int esr_mrhs_steps_ell(int64_t n, int64_t K, const uint8_t *len,
+                       const int32_t *ecol, const double *eval, const double *Av,
+                       const double *diag, double *X, double *R, double *Z,
+                       double *P, double *AP, void *state, double *ws,
+                       int32_t *active, int64_t count, void *stream);
 
 /* ---- learned preconditioner (new; the reference's preconditioner slot,
  * easier/numeric/preconditioner/__init__.py:4, is empty) --------------------

@@ -73,6 +73,28 @@ def test_batched_many_rhs_one_mesh(gpu, K):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize('jitter', [0.0, 0.2])
+def test_multi_rhs_ell_product_gives_the_csr_bits(gpu, monkeypatch, jitter):
+    """The shared-operator solver on the padded ELL copy of the operator
+    (rows of at most 8 entries) and on the CSR: the same iterations,
+    residuals and solution bits, for every right-hand side."""
+    from easier_amd.numeric.solver.batched import MultiRhsCG
+    mesh = wmesh.tri2(37, jitter=jitter, seed=3)
+    systems = [wp.p1_system(mesh, None, wp.rhs_uniform(mesh['points'].shape[0], k))
+               for k in range(13)]
+    out = {}
+    for ell in (True, False):
+        monkeypatch.setattr(MultiRhsCG, 'use_ell', ell)
+        bcg = BatchedCG(systems)
+        assert bcg.multi is not None and (bcg.multi.ell is not None) == ell
+        out[ell] = (bcg.solve(rtol=1e-10), bcg.solutions())
+    assert out[True][0] == out[False][0]
+    for a, b in zip(out[True][1], out[False][1]):
+        assert np.array_equal(a, b)
+    info_o, x_o = _oracle_solve(systems[5], 1e-10)
+    assert abs(out[True][0][5]['iters'] - info_o['iters']) <= 1
+
+
 def test_batched_split_autotuner_keeps_every_rhs(gpu):
     """The shared-operator autotuner (1, 2 or 4 groups of right-hand sides)
     picks a split by timing; whichever it picks, each right-hand side keeps
