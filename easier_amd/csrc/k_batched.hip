@@ -102,6 +102,12 @@ __global__ __launch_bounds__(kRB) void k_b_spmv(
   store_partials<1>(acc, part);
 }
 
+// (The product on a padded row-major ELL copy of the concatenated operator
+// -- 8 slots per row, no row-pointer loads, as the multi-right-hand-side
+// solver reads it -- gave the same bits but took 1.54 s against 1.51 s for
+// the 256 meshes of sides 32-512: with a row per thread the padding's bytes
+// cost more than the row pointers.)
+
 // Step part 2 (cg.py:47-52) + first half of Update: alpha, x, r, z;
 // partials of r.r and r.z.
 __global__ __launch_bounds__(kRB) void k_b_xr(Tiles tl, const double *__restrict__ diag,
