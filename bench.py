@@ -654,6 +654,9 @@ def main():
                     help='only the Jacobi-PCG iteration of the 2048 x 2048 variable-'
                          'coefficient system in --dtype, 3 captured chunks of 32 '
                          'iterations after one warm-up chunk (for rocprofv3 runs)')
+    ap.add_argument('--config3-only', action='store_true',
+                    help='only the batched configuration (64 right-hand sides on 1024^2, '
+                         '256 meshes), as in the default run (for rocprofv3 runs)')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy a float64 array of MIB MiB 5 times (the known-size '
                          'kernel that scales the HBM byte counters)')
@@ -668,6 +671,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.config3_only:
+        print(json.dumps(batched_config3()), flush=True)
         return
     if args.pcg_profile:
         print(json.dumps(pcg_profile(args.dtype)), flush=True)

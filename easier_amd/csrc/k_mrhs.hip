@@ -304,6 +304,9 @@ __global__ __launch_bounds__(kRB) void k_m_test(int K, int64_t g, const double *
   s->rz_next = rz[0];
 }
 
+// (Two rows per loop iteration in this kernel and k_m_xr, both rows' loads
+// in flight together, measured 2 % slower: 5.21 vs 5.11 s for 64 right-hand
+// sides on 1024^2.)
 // The step's x += alpha p (cg.py:46, for every side that ran this step,
 // including one the loop test has just stopped) and Update (cg.py:67-71):
 // p = z + beta p with z = M r, for the sides still running.
