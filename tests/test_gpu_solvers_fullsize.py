@@ -4,7 +4,8 @@ relative tolerance 1e-8. The oracle's CG would need minutes on the host, so
 the checks are size-independent: the true residual ||b - A x|| recomputed on
 the host with scipy, and two independent methods (fused Jacobi-PCG, the
 three-kernel form at this size, and multigrid flexible CG with damped-Jacobi
-or the trained network as smoother) agreeing on x."""
+or the trained network as smoother, on the geometric hierarchy and on the
+smoothed-aggregation one) agreeing on x."""
 import os
 
 import pytest
@@ -20,8 +21,8 @@ pytestmark = pytest.mark.gpu
 def test_full_size_jacobi_pcg_and_multigrid_fcg_agree(gpu):
     print('full-size Poisson 2048x2048: Jacobi-PCG and multigrid FCG', flush=True)
     res = wl.evaluate_compiled(lp.load(WEIGHTS), 2048, 1, rtol=1e-8, maxiter=20000, reps=1,
-                               methods=('jacobi', 'mg', 'mg-learned'))
-    jac, mg, ml = res['jacobi'], res['mg'], res['mg-learned']
+                               methods=('jacobi', 'mg', 'mg-learned', 'amg'))
+    jac, mg, ml, amg = res['jacobi'], res['mg'], res['mg-learned'], res['amg']
     assert ml['true_rel_residual'] <= 1e-8 * (1 + 1e-6), ml
     assert ml['rel_diff_x'] <= 1e-6 and ml['iters'] <= 20 and ml['fallback_at'] is None, ml
     assert res['vertices'] == 2049 * 2049
@@ -31,3 +32,5 @@ def test_full_size_jacobi_pcg_and_multigrid_fcg_agree(gpu):
     # number ~1e7) that leaves x within ~1e-6 of each other
     assert mg['rel_diff_x'] <= 1e-6, mg
     assert 5000 <= jac['iters'] <= 12000 and mg['iters'] <= 20 and mg['fallback_at'] is None
+    assert amg['true_rel_residual'] <= 1e-8 * (1 + 1e-6), amg
+    assert amg['rel_diff_x'] <= 1e-6 and amg['iters'] <= 30 and amg['fallback_at'] is None, amg
